@@ -1,0 +1,174 @@
+"""Benchmark of the matrix-free Q2 A11 viscous apply on one MI355X.
+
+One step = one FP64 A11 apply y = A11 u over the whole 128^3-element Q2 mesh
+(50,923,779 velocity dofs): 8 coloured launches of the k_a11_fp64 brick kernel.
+Inputs are resident in HBM before the timed region.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for p in (REPO, os.path.join(REPO, "ptatin-gene_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes_a11(m, value_bytes=8):
+    """Canonical bytes of one A11 apply: u read, y written, coordinates, eta, and an int32
+    connectivity entry per element node (each array counted once)."""
+    n3 = 3 * (2 * m + 1) ** 3
+    e = m ** 3
+    return value_bytes * n3 + value_bytes * n3 + 8 * n3 + value_bytes * 27 * e + 4 * 27 * e
+
+
+def build_problem(m):
+    from ptq2 import mesh as M
+
+    # configs[1] recipe (affine+jitter mesh, 12 inclusions eta 1e3, log-uniform noise) at m^3
+    return M.config_sinker(m, n_incl=12, contrast=1e3, bc="none")
+
+
+def time_applies(fn, steps, warmup, stream):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record(stream)
+    for _ in range(steps):
+        fn()
+    t1.record(stream)
+    torch.cuda.synchronize()
+    return t0.elapsed_time(t1) / steps  # ms per step
+
+
+def cpu_baseline(seconds=12.0):
+    """The oracle's restatement of the reference's default sum-factorized CPU algorithm
+    (OpenMP over race-free element colours) on a bounded 40^3 sample."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from ptq2 import mesh as M
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    m = 40
+    P = M.config_sinker(m, n_incl=12, contrast=1e3, bc="none")
+    u = M.uniform(2, P.nvel, -1.0, 1.0)
+    O.a11_tensor(m, m, m, P.coords, P.eta, u, threads=threads)  # warm
+    n, t_start = 0, time.perf_counter()
+    while True:
+        O.a11_tensor(m, m, m, P.coords, P.eta, u, threads=threads)
+        n += 1
+        el = time.perf_counter() - t_start
+        if el >= seconds:
+            break
+    t = el / n
+    return {"value": round(P.nvel / t / 1e9, 5), "unit": "GDoF/s", "cores": threads, "kind": "port",
+            "sample": "%d FP64 A11 applies on a 40^3-element Q2 mesh (%d velocity dofs), %.1f s" % (n, P.nvel, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fp32", action="store_true", help="also time the FP32 smoother operator")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build runs on a single MI355X; --gpus %d is not supported"
+              % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ptq2
+    from ptq2 import _lib
+
+    m = args.size
+    t_setup = time.perf_counter()
+    P = build_problem(m)
+    dev = torch.device("cuda:0")
+    A = ptq2.MatA11MF("hip", device=0)
+    A.setup(P)
+    u = torch.from_numpy(ptq2.mesh.uniform(2, P.nvel, -1.0, 1.0)).to(dev)
+    y = torch.empty_like(u)
+    stream = torch.cuda.current_stream()
+    setup_s = time.perf_counter() - t_setup
+    print("setup %.1f s (%d elements, %d dofs)" % (setup_s, P.nel, P.nvel), file=sys.stderr, flush=True)
+
+    ms = time_applies(lambda: A.ctx.a11(u, y, 0), args.steps, args.warmup, stream)
+    gdofs = P.nvel / (ms * 1e-3) / 1e9
+    bytes_alg = algorithmic_bytes_a11(m)
+    achieved = bytes_alg / (ms * 1e-3) / 1e9
+    launches = 8
+    extra = {}
+    if args.fp32:
+        u32 = u.float()
+        y32 = torch.empty_like(u32)
+        A.ctx.a11(u32, y32, _lib.FP32)  # builds the FP32 factors outside the timed region
+        ms32 = time_applies(lambda: A.ctx.a11(u32, y32, _lib.FP32), args.steps, args.warmup, stream)
+        extra["fp32_gdofs"] = round(P.nvel / (ms32 * 1e-3) / 1e9, 3)
+        extra["fp32_ms"] = round(ms32, 4)
+        torch.cuda.synchronize()
+        extra["fp32_rel_err_vs_fp64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
+
+    traffic = None
+    prof = os.path.join(REPO, "profiles", "a11_fp64_128_traffic.json")
+    if m == 128 and os.path.exists(prof):
+        with open(prof) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": "Q2 A11 apply GDoF/s (%HBM peak)",
+        "value": round(gdofs, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "A11 FP64 matrix-free apply y=A11 u, %d^3 Q2 elements (%d velocity dofs), affine+jitter mesh, "
+                        "12 sphere inclusions eta=1e3 with log-uniform noise per quadrature point" % (m, P.nvel),
+            "elements": P.nel,
+            "velocity_dofs": P.nvel,
+            "launches_per_step": launches,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+        },
+        "cpu_baseline": cpu,
+    }
+    if extra:
+        out["extra"] = extra
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

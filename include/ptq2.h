@@ -1,0 +1,112 @@
+/*
+ * ptq2.h -- C ABI of the MI355X matrix-free Q2-P1disc Stokes operator library
+ * (libptq2.so).  Plain C types only: the PETSc-side adapter
+ * (ptatin-gene_amd/adapter/stokes_operators_hip.c), the solver host code and the
+ * tests bind to exactly these entry points.
+ *
+ * Mesh: one structured Q2 node box of NX x NY x NZ nodes holding mx x my x mz
+ * owned elements whose first node sits at (i0,j0,k0) of the box; node
+ * n = i + NX*(j + NY*k), interlaced velocity dofs 3n+l, element
+ * e = ex + mx*(ey + my*ez), pressure dofs 4e+d, quadrature point q = a+3b+9c.
+ * A single-rank DMDA is NX=2mx+1, (i0,j0,k0)=(0,0,0); a rank's ghosted local
+ * box is the general case.  This is the numbering of DMDAGetElements_DA_Q2_3D
+ * (src/dmda_element_q2p1.c:368-469) and DMDAGetElements_DA_P0MD_3D (:491-546).
+ *
+ * Every function returns PTQ2_OK (0) or a nonzero PTQ2_ERR_* code; no function
+ * aborts the process.  A context is not thread safe; independent contexts are.
+ */
+#ifndef PTQ2_H
+#define PTQ2_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ptq2_ctx ptq2_ctx;
+
+enum {
+  PTQ2_OK = 0,
+  PTQ2_ERR_INVALID_ARG = 1, /* null pointer, bad flag or bad dimension */
+  PTQ2_ERR_NOT_SETUP = 2,   /* mesh or viscosity missing (SETERRQ "MF operator not setup") */
+  PTQ2_ERR_SIZE = 3,        /* size overflow or above the supported maximum */
+  PTQ2_ERR_ALLOC = 4,       /* device or host allocation failed */
+  PTQ2_ERR_DEVICE = 5,      /* HIP runtime error */
+  PTQ2_ERR_NONFINITE = 6,   /* NaN or Inf in coordinates, viscosity or input vector */
+  PTQ2_ERR_MESH = 7,        /* non-positive Jacobian determinant at a quadrature point */
+  PTQ2_ERR_VISCOSITY = 8,   /* eta <= 0 at a quadrature point */
+  PTQ2_ERR_UNSUPPORTED = 9  /* connectivity that is not a structured DMDA box */
+};
+
+/* flags of the apply entry points */
+#define PTQ2_DEVICE 0x1u  /* vectors are device pointers; the call is stream ordered and
+                             returns before the work completes (ptq2_sync to wait) */
+#define PTQ2_FP32 0x2u    /* float vectors; A11 runs the FP32 smoother operator */
+#define PTQ2_MATMULT 0x4u /* MatShell semantics: Dirichlet columns zeroed on input and
+                             Dirichlet rows replaced on output */
+#define PTQ2_ADD 0x8u     /* y += (element sum), the SpMVOp raw-kernel contract;
+                             without it y = (element sum) */
+
+/* lifecycle ------------------------------------------------------------- */
+/* replaces MFA11CUDA_SetUp (include/stokes_operators.h:124-149) and the
+   SpMVOp_SetUp hook of struct _p_MatA11MF (include/stokes_operators.h:48-64) */
+int ptq2_create(ptq2_ctx **ctx, int device);
+/* replaces MFA11CUDA_CleanUp / SpMVOp_Destroy */
+int ptq2_destroy(ptq2_ctx **ctx);
+const char *ptq2_strerror(int code);
+/* message of the last failing call on this context ("" when none) */
+const char *ptq2_last_error(const ptq2_ctx *ctx);
+/* run subsequent work on a caller-owned hipStream_t (NULL: the context's own) */
+int ptq2_set_stream(ptq2_ctx *ctx, void *hip_stream);
+int ptq2_sync(ptq2_ctx *ctx);
+
+/* mesh and coefficients ---------------------------------------------------- */
+/* Uploads the coordinates (interlaced FP64, 3*NX*NY*NZ) once; they stay resident.
+   elnidx (27*mx*my*mz, may be NULL) is checked against the structured map of the
+   box; dirichlet_mask (3*NX*NY*NZ bytes, nonzero = Dirichlet dof, may be NULL) is
+   the BCList of src/dmda_bcs.c.  Replaces the connectivity/coordinate uploads of
+   CopyTo_A11_CUDA (src/stokes_operators_cuda.cu:325-431). */
+int ptq2_setup_mesh(ptq2_ctx *ctx, int mx, int my, int mz, const double *coords,
+                    const int32_t *elnidx, const uint8_t *dirichlet_mask);
+int ptq2_setup_mesh_box(ptq2_ctx *ctx, int mx, int my, int mz, int NX, int NY, int NZ,
+                        int i0, int j0, int k0, const double *coords,
+                        const uint8_t *dirichlet_mask);
+/* Quadrature-point viscosity eta_qp[27*e+q] (the .eta of QPntVolCoefStokes,
+   include/QPntVolCoefStokes_def.h:41-46).  `state` mirrors mf->state
+   (src/stokes_operators.c:1176-1177): an upload with the state already cached is
+   skipped.  flags may hold PTQ2_DEVICE.  The FP32 operator's geometric factors
+   (FP64 Jacobian, then rounded) are rebuilt on the next FP32 apply. */
+int ptq2_set_viscosity(ptq2_ctx *ctx, const double *eta_qp, uint64_t state, unsigned flags);
+
+/* operators ----------------------------------------------------------------- */
+/* A11 viscous block.  Raw mode (no PTQ2_MATMULT) is the SpMVOp_MatMult contract of
+   MFStokesWrapper_A11_* (include/stokes_operators_mf.h:36-70):
+   ufield[] local velocity in, element contributions summed into Yu[].
+   PTQ2_MATMULT is MatMult_MFStokes_A11 (src/stokes_operators.c:1156-1222). */
+int ptq2_a11_apply(ptq2_ctx *ctx, const void *u, void *y, unsigned flags);
+/* diagonal of A11: MFStokesWrapper_diagA11 (src/stokes_operators_mf.c:71-135);
+   with PTQ2_MATMULT, MatGetDiagonal_MFStokes_A11 (src/stokes_operators.c:1072-1113)
+   which puts 1 in Dirichlet slots.  FP64 only. */
+int ptq2_a11_diag(ptq2_ctx *ctx, double *d, unsigned flags);
+/* gradient A12 (pressure 4E in, velocity out): MFStokesWrapper_A12_AVX
+   (src/stokes_operators_avx.c:682-829); PTQ2_MATMULT is MatMult_MFStokes_A12
+   (src/stokes_operators.c:1301-1370): Dirichlet rows of y set to 0. */
+int ptq2_a12_apply(ptq2_ctx *ctx, const double *p, double *yu, unsigned flags);
+/* divergence A21 (velocity in, pressure 4E out): MFStokesWrapper_A21
+   (src/stokes_operators_mf.c:862-937); PTQ2_MATMULT is MatMult_MFStokes_A21
+   (src/stokes_operators.c:1444-1509): Dirichlet columns of u zeroed. */
+int ptq2_a21_apply(ptq2_ctx *ctx, const double *u, double *yp, unsigned flags);
+/* full block [A11 A12; A21 0]: MFStokesWrapper_A_AVX (src/stokes_operators_avx.c:439-636);
+   PTQ2_MATMULT is MatMult_MFStokes_A (src/stokes_operators.c:893-979). */
+int ptq2_stokes_apply(ptq2_ctx *ctx, const double *u, const double *p, double *yu,
+                      double *yp, unsigned flags);
+
+/* introspection (sizes, for callers that allocate device vectors) */
+int ptq2_sizes(const ptq2_ctx *ctx, int64_t *n_velocity_dofs, int64_t *n_pressure_dofs,
+               int64_t *n_elements);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTQ2_H */

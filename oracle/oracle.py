@@ -1,0 +1,187 @@
+"""ctypes binding of the CPU oracle (liboracle built from ptq2_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and only there as the checker.  The product
+library never loads it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libptq2_oracle.so")
+_lib = None
+
+_D = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_I = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_U8 = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+_i = ctypes.c_int
+_d = ctypes.c_double
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        sig = {
+            "ptq2o_mesh_uniform": [_i, _i, _i, _d, _d, _d, _d, _d, _d, _D],
+            "ptq2o_mesh_bump_ymax": [_i, _i, _i, _d, _d, _D],
+            "ptq2o_elnidx": [_i, _i, _i, _I],
+            "ptq2o_bc_mask": [_i, _i, _i, _i, _U8],
+            "ptq2o_test_vector_u": [_i, _i, _i, _D, _D],
+            "ptq2o_test_vector_p": [_i, _i, _i, _i, _D],
+            "ptq2o_a11_tensor": [_i, _i, _i, _D, _D, _D, _D],
+            "ptq2o_a11_tensor_omp": [_i, _i, _i, _D, _D, _D, _D, _i],
+            "ptq2o_a11_ref": [_i, _i, _i, _D, _D, _D, _D],
+            "ptq2o_a12": [_i, _i, _i, _D, _D, _D],
+            "ptq2o_a21": [_i, _i, _i, _D, _D, _D],
+            "ptq2o_diag_a11": [_i, _i, _i, _D, _D, _D],
+            "ptq2o_stokes_a": [_i, _i, _i, _D, _D, _D, _D, _D, _D],
+            "ptq2o_matmult_a11": [_i, _i, _i, _D, _D, _U8, _D, _D],
+            "ptq2o_matmult_a12": [_i, _i, _i, _D, _U8, _D, _D],
+            "ptq2o_matmult_a21": [_i, _i, _i, _D, _U8, _D, _D],
+            "ptq2o_matgetdiag_a11": [_i, _i, _i, _D, _D, _U8, _D],
+            "ptq2o_matmult_a": [_i, _i, _i, _D, _D, _U8, _D, _D, _D, _D],
+            "ptq2o_a11_element_matrix": [_D, _D, _D],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        _lib = L
+    return _lib
+
+
+def nnodes(mx, my, mz):
+    return (2 * mx + 1) * (2 * my + 1) * (2 * mz + 1)
+
+
+def mesh_uniform(mx, my, mz, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0)):
+    c = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_mesh_uniform(mx, my, mz, *box, c)
+    return c
+
+
+def mesh_bump(mx, my, mz, coords, amp=-0.3, lam=-5.6):
+    c = np.array(coords, dtype=np.float64, copy=True)
+    lib().ptq2o_mesh_bump_ymax(mx, my, mz, amp, lam, c)
+    return c
+
+
+def elnidx(mx, my, mz):
+    e = np.zeros(27 * mx * my * mz, dtype=np.int32)
+    lib().ptq2o_elnidx(mx, my, mz, e)
+    return e
+
+
+def bc_mask(mx, my, mz, kind=0):
+    m = np.zeros(3 * nnodes(mx, my, mz), dtype=np.uint8)
+    lib().ptq2o_bc_mask(mx, my, mz, kind, m)
+    return m
+
+
+def test_vector_u(mx, my, mz, coords):
+    u = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_test_vector_u(mx, my, mz, np.ascontiguousarray(coords), u)
+    return u
+
+
+def test_vector_p(mx, my, mz, legacy=True):
+    p = np.zeros(4 * mx * my * mz)
+    lib().ptq2o_test_vector_p(mx, my, mz, 1 if legacy else 0, p)
+    return p
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def a11_tensor(mx, my, mz, coords, eta, u, threads=0):
+    y = np.zeros(3 * nnodes(mx, my, mz))
+    if threads:
+        lib().ptq2o_a11_tensor_omp(mx, my, mz, _f64(coords), _f64(eta), _f64(u), y, threads)
+    else:
+        lib().ptq2o_a11_tensor(mx, my, mz, _f64(coords), _f64(eta), _f64(u), y)
+    return y
+
+
+def a11_ref(mx, my, mz, coords, eta, u):
+    y = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_a11_ref(mx, my, mz, _f64(coords), _f64(eta), _f64(u), y)
+    return y
+
+
+def a12(mx, my, mz, coords, p):
+    y = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_a12(mx, my, mz, _f64(coords), _f64(p), y)
+    return y
+
+
+def a21(mx, my, mz, coords, u):
+    y = np.zeros(4 * mx * my * mz)
+    lib().ptq2o_a21(mx, my, mz, _f64(coords), _f64(u), y)
+    return y
+
+
+def diag_a11(mx, my, mz, coords, eta):
+    d = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_diag_a11(mx, my, mz, _f64(coords), _f64(eta), d)
+    return d
+
+
+def stokes_a(mx, my, mz, coords, eta, u, p):
+    yu = np.zeros(3 * nnodes(mx, my, mz))
+    yp = np.zeros(4 * mx * my * mz)
+    lib().ptq2o_stokes_a(mx, my, mz, _f64(coords), _f64(eta), _f64(u), _f64(p), yu, yp)
+    return yu, yp
+
+
+def _mask(mask, n):
+    if mask is None:
+        return np.zeros(n, dtype=np.uint8)
+    return np.ascontiguousarray(mask, dtype=np.uint8)
+
+
+def matmult_a11(mx, my, mz, coords, eta, mask, x):
+    y = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_matmult_a11(mx, my, mz, _f64(coords), _f64(eta), _mask(mask, y.size), _f64(x), y)
+    return y
+
+
+def matmult_a12(mx, my, mz, coords, mask, p):
+    y = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_matmult_a12(mx, my, mz, _f64(coords), _mask(mask, y.size), _f64(p), y)
+    return y
+
+
+def matmult_a21(mx, my, mz, coords, mask, u):
+    y = np.zeros(4 * mx * my * mz)
+    lib().ptq2o_matmult_a21(mx, my, mz, _f64(coords), _mask(mask, u.size), _f64(u), y)
+    return y
+
+
+def matgetdiag_a11(mx, my, mz, coords, eta, mask):
+    d = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_matgetdiag_a11(mx, my, mz, _f64(coords), _f64(eta), _mask(mask, d.size), d)
+    return d
+
+
+def matmult_a(mx, my, mz, coords, eta, mask, u, p):
+    yu = np.zeros(3 * nnodes(mx, my, mz))
+    yp = np.zeros(4 * mx * my * mz)
+    lib().ptq2o_matmult_a(mx, my, mz, _f64(coords), _f64(eta), _mask(mask, yu.size), _f64(u), _f64(p), yu, yp)
+    return yu, yp
+
+
+def a11_element_matrix(elcoords, eta27):
+    K = np.zeros(81 * 81)
+    lib().ptq2o_a11_element_matrix(_f64(elcoords), _f64(eta27), K)
+    return K.reshape(81, 81)

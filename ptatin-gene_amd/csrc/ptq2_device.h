@@ -1,0 +1,362 @@
+// ptq2_device.h -- device-side building blocks shared by the operator kernels.
+//
+// Work decomposition (MI355X, gfx950): one 64-lane wavefront owns a brick of
+// 4x4x4 Q2 elements, lane = element.  The brick's 9x9x9 node tile of
+// coordinates and velocity is staged once in LDS with row-contiguous global
+// loads; every lane then runs the whole sum-factorized element computation in
+// registers (no cross-lane traffic), accumulates its 81 outputs into an LDS
+// output tile with ds_add_f64 (each instruction's 64 addresses are distinct,
+// so the sum order is the fixed program order), and the tile is written back
+// with row-contiguous stores.  Bricks are launched in 8 colours
+// (brick parity in x, y, z): bricks of one colour share no node, so the
+// write-back is a plain load-add-store and the result is bitwise reproducible.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ptq2 {
+
+constexpr int BE = 4;               // elements per brick edge
+constexpr int BEL = BE * BE * BE;   // elements per brick (= lanes per wave)
+constexpr int TN = 2 * BE + 1;      // tile nodes per edge
+constexpr int TROW = TN;            // node stride between tile rows
+constexpr int TPL = TN * TN;        // node stride between tile planes
+constexpr int TN3 = TN * TN * TN;   // nodes per tile
+constexpr int TCS = TN3 + 3;        // component stride in a tile (padded)
+constexpr int TROWS = TN * TN;      // rows per tile
+constexpr int RLEN = 3 * TN;        // doubles per row (interlaced nodes)
+
+// Mesh box: NX x NY x NZ nodes, mx x my x mz owned elements, first node of
+// element (0,0,0) at (i0,j0,k0); bricks nbx x nby x nbz.
+struct Box {
+  int mx, my, mz;
+  int NX, NY, NZ;
+  int i0, j0, k0;
+  int nbx, nby, nbz;
+};
+
+// 1D Q2 Lagrange basis on nodes {-1,0,1} at the Gauss points {-sqrt(3/5),0,sqrt(3/5)}:
+// B(q,n) = l_n(x_q), D(q,n) = l_n'(x_q) (src/stokes_operators_tensor.c:193-206).
+__host__ __device__ constexpr double gx(int q) { return q == 0 ? -0.77459666924148337704 : (q == 1 ? 0.0 : 0.77459666924148337704); }
+__host__ __device__ constexpr double gB(int q, int n) {
+  return n == 0 ? 0.5 * (gx(q) * gx(q) - gx(q)) : (n == 1 ? 1.0 - gx(q) * gx(q) : 0.5 * (gx(q) * gx(q) + gx(q)));
+}
+__host__ __device__ constexpr double gD(int q, int n) {
+  return n == 0 ? gx(q) - 0.5 : (n == 1 ? -2.0 * gx(q) : gx(q) + 0.5);
+}
+__host__ __device__ constexpr double gw1(int q) { return q == 1 ? 8.0 / 9.0 : 5.0 / 9.0; }
+// The exact zeros of the tables: B(1,0)=B(1,2)=0 and D(1,1)=0.
+__host__ __device__ constexpr bool nzB(int q, int n) { return !(q == 1 && n != 1); }
+__host__ __device__ constexpr bool nzD(int q, int n) { return !(q == 1 && n == 1); }
+__host__ __device__ constexpr bool isoneB(int q, int n) { return q == 1 && n == 1; }
+
+template <typename T>
+__device__ __forceinline__ T fmac(double c, T x, T acc) {
+  return fma(static_cast<T>(c), x, acc);
+}
+
+// brick of a coloured launch
+struct BrickId {
+  int bx, by, bz, lin;
+};
+__device__ __forceinline__ BrickId brick_of(const Box &b, int color, int bid) {
+  int cx = color & 1, cy = (color >> 1) & 1, cz = (color >> 2) & 1;
+  int ncx = (b.nbx - cx + 1) >> 1, ncy = (b.nby - cy + 1) >> 1;
+  BrickId r;
+  r.bx = cx + 2 * (bid % ncx);
+  r.by = cy + 2 * ((bid / ncx) % ncy);
+  r.bz = cz + 2 * (bid / (ncx * ncy));
+  r.lin = r.bx + b.nbx * (r.by + b.nby * r.bz);
+  return r;
+}
+// brick of an uncoloured launch (outputs not shared between bricks)
+__device__ __forceinline__ BrickId brick_lin(const Box &b, int bid) {
+  BrickId r;
+  r.bx = bid % b.nbx;
+  r.by = (bid / b.nbx) % b.nby;
+  r.bz = bid / (b.nbx * b.nby);
+  r.lin = bid;
+  return r;
+}
+
+struct Tile {
+  int ex0, ey0, ez0;   // first element of the brick
+  int nex, ney, nez;   // elements in the brick per direction
+  int64_t node0;       // box node index of tile node (0,0,0)
+  int tnx, tny, tnz;   // tile nodes present per direction
+};
+__device__ __forceinline__ Tile tile_of(const Box &b, const BrickId &br) {
+  Tile t;
+  t.ex0 = br.bx * BE; t.ey0 = br.by * BE; t.ez0 = br.bz * BE;
+  t.nex = min(BE, b.mx - t.ex0); t.ney = min(BE, b.my - t.ey0); t.nez = min(BE, b.mz - t.ez0);
+  t.tnx = 2 * t.nex + 1; t.tny = 2 * t.ney + 1; t.tnz = 2 * t.nez + 1;
+  t.node0 = (int64_t)(b.i0 + 2 * t.ex0) + (int64_t)b.NX * ((int64_t)(b.j0 + 2 * t.ey0) + (int64_t)b.NY * (int64_t)(b.k0 + 2 * t.ez0));
+  return t;
+}
+
+// Stage an interlaced 3-component nodal field into an LDS tile [comp][z][y][x].
+// Rows of the tile are contiguous runs of 27 values in global memory.  When mask
+// is given, Dirichlet dofs read as zero (BCListInsertLocalZero,
+// src/dmda_bcs.c:581-608).  Tile nodes outside the box read as zero.
+template <typename T, typename S>
+__device__ __forceinline__ void load_tile(const Box &b, const Tile &t, const S *__restrict__ g,
+                                          const uint8_t *__restrict__ mask, T *__restrict__ s, int lane) {
+  for (int f = lane; f < TROWS * RLEN; f += BEL) {
+    int r = f / RLEN, col = f - r * RLEN;
+    int jj = r % TN, kk = r / TN;
+    int ii = col / 3, c = col - 3 * ii;
+    T v = T(0);
+    if (ii < t.tnx && jj < t.tny && kk < t.tnz) {
+      int64_t gi = 3 * (t.node0 + ii + (int64_t)b.NX * (jj + (int64_t)b.NY * kk)) + c;
+      v = static_cast<T>(g[gi]);
+      if (mask && mask[gi]) v = T(0);
+    }
+    s[c * TCS + kk * TPL + jj * TROW + ii] = v;
+  }
+}
+
+// Write an LDS output tile back.  Bricks of one colour never share a node.  In
+// overwrite mode the first brick to touch a node (parity 0 along every axis on
+// which the node is shared, i.e. the lowest colour) stores, the later colours add.
+template <typename T>
+__device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, const Tile &t, const T *__restrict__ s,
+                                           T *__restrict__ g, bool add_mode, int lane) {
+  for (int f = lane; f < TROWS * RLEN; f += BEL) {
+    int r = f / RLEN, col = f - r * RLEN;
+    int jj = r % TN, kk = r / TN;
+    int ii = col / 3, c = col - 3 * ii;
+    if (ii < t.tnx && jj < t.tny && kk < t.tnz) {
+      int64_t gi = 3 * (t.node0 + ii + (int64_t)b.NX * (jj + (int64_t)b.NY * kk)) + c;
+      T v = s[c * TCS + kk * TPL + jj * TROW + ii];
+      bool shx = (ii == 0 && br.bx > 0) || (ii == 2 * BE && br.bx < b.nbx - 1);
+      bool shy = (jj == 0 && br.by > 0) || (jj == 2 * BE && br.by < b.nby - 1);
+      bool shz = (kk == 0 && br.bz > 0) || (kk == 2 * BE && br.bz < b.nbz - 1);
+      bool first = (!shx || !(br.bx & 1)) && (!shy || !(br.by & 1)) && (!shz || !(br.bz & 1));
+      if (add_mode || !first) v += g[gi];
+      g[gi] = v;
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void zero_tile(T *__restrict__ s, int lane) {
+  for (int f = lane; f < 3 * TCS; f += BEL) s[f] = T(0);
+}
+
+__device__ __forceinline__ void lds_add(double *p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_add(float *p, float v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// lane -> element inside the brick and its LDS tile base offset
+__device__ __forceinline__ int lane_base(int lane) {
+  int lx = lane & (BE - 1), ly = (lane >> 2) & (BE - 1), lz = lane >> 4;
+  return 2 * lz * TPL + 2 * ly * TROW + 2 * lx;
+}
+__device__ __forceinline__ bool lane_active(const Tile &t, int lane) {
+  int lx = lane & (BE - 1), ly = (lane >> 2) & (BE - 1), lz = lane >> 4;
+  return lx < t.nex && ly < t.ney && lz < t.nez;
+}
+__device__ __forceinline__ int64_t lane_element(const Box &b, const Tile &t, int lane) {
+  int lx = lane & (BE - 1), ly = (lane >> 2) & (BE - 1), lz = lane >> 4;
+  return (int64_t)(t.ex0 + lx) + (int64_t)b.mx * ((int64_t)(t.ey0 + ly) + (int64_t)b.my * (int64_t)(t.ez0 + lz));
+}
+
+// Tile value at element-local node (i,j,k), component c.
+template <typename T>
+__device__ __forceinline__ T tv(const T *s, int base, int c, int i, int j, int k) {
+  return s[c * TCS + base + k * TPL + j * TROW + i];
+}
+
+// Row data of one field for quadrature row (qb,qc): for each component c and
+// element-local node column i,
+//   r0[c][i] = sum_jk B(qb,j) B(qc,k) F,  r1 = sum D(qb,j) B(qc,k) F,  r2 = sum B(qb,j) D(qc,k) F.
+template <int qb, int qc, typename T>
+__device__ __forceinline__ void row_data(const T *s, int base, T r0[3][3], T r1[3][3], T r2[3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      T a0 = T(0), a1 = T(0), a2 = T(0);
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+          const bool u0 = nzB(qb, j) && nzB(qc, k);
+          const bool u1 = nzD(qb, j) && nzB(qc, k);
+          const bool u2 = nzB(qb, j) && nzD(qc, k);
+          if (u0 || u1 || u2) {
+            T v = tv(s, base, c, i, j, k);
+            if (u0) a0 = fmac(gB(qb, j) * gB(qc, k), v, a0);
+            if (u1) a1 = fmac(gD(qb, j) * gB(qc, k), v, a1);
+            if (u2) a2 = fmac(gB(qb, j) * gD(qc, k), v, a2);
+          }
+        }
+      r0[c][i] = a0; r1[c][i] = a1; r2[c][i] = a2;
+    }
+}
+
+// Reference-gradient at quadrature point (qa,qb,qc) from the row data:
+// g[d][c] = d F_c / d xi_d.
+template <int qa, typename T>
+__device__ __forceinline__ void qp_grad(const T r0[3][3], const T r1[3][3], const T r2[3][3], T g[3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T a0 = T(0), a1 = T(0), a2 = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      if (nzD(qa, i)) a0 = fmac(gD(qa, i), r0[c][i], a0);
+      if (nzB(qa, i)) a1 = fmac(gB(qa, i), r1[c][i], a1);
+      if (nzB(qa, i)) a2 = fmac(gB(qa, i), r2[c][i], a2);
+    }
+    g[0][c] = a0; g[1][c] = a1; g[2][c] = a2;
+  }
+}
+
+// value at quadrature point (qa,...) from the B-B row data r0
+template <int qa, typename T>
+__device__ __forceinline__ void qp_value(const T r0[3][3], T v[3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    T a = T(0);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+      if (nzB(qa, i)) a = fmac(gB(qa, i), r0[c][i], a);
+    v[c] = a;
+  }
+}
+
+// Adjugate of J (J[d][l] = dx_l/dxi_d) arranged so that inv(J)[k][m] = A[k][m]/det,
+// i.e. A[k][m] = d xi_m / d x_k * det.  Same cofactors as JacobianInvertNEV
+// (src/stokes_operators_tensor.c:94-125).
+template <typename T>
+__device__ __forceinline__ T adjugate(const T J[3][3], T A[3][3]) {
+  A[0][0] = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+  A[1][0] = -(J[1][0] * J[2][2] - J[2][0] * J[1][2]);
+  A[2][0] = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+  A[0][1] = -(J[0][1] * J[2][2] - J[2][1] * J[0][2]);
+  A[1][1] = J[0][0] * J[2][2] - J[2][0] * J[0][2];
+  A[2][1] = -(J[0][0] * J[2][1] - J[2][0] * J[0][1]);
+  A[0][2] = J[0][1] * J[1][2] - J[1][1] * J[0][2];
+  A[1][2] = -(J[0][0] * J[1][2] - J[1][0] * J[0][2]);
+  A[2][2] = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+  return J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+}
+
+// FP64 reciprocal: hardware estimate refined by two Newton steps (error ~1 ulp).
+__device__ __forceinline__ double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float rcp32(float x) { return 1.0f / x; }
+
+// Stress action at one quadrature point.  With M[k][l] = sum_m g[m][l] A[k][m]
+// (= det * du_l/dx_k) and S = s (M + M^T), s = w eta / det, the test-function
+// coefficient is dv[k][l] = sum_m S[m][l] A[m][k]; this equals w det J^-T (2 eta eps(u))
+// of QuadratureAction (src/stokes_operators_tensor.c:127-179).
+template <typename T>
+__device__ __forceinline__ void stress_action(const T A[3][3], const T g[3][3], T s, T dv[3][3]) {
+  T M[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) M[k][l] = g[0][l] * A[k][0] + g[1][l] * A[k][1] + g[2][l] * A[k][2];
+  T S[3][3];
+  T s2 = s + s;
+  S[0][0] = s2 * M[0][0];
+  S[1][1] = s2 * M[1][1];
+  S[2][2] = s2 * M[2][2];
+  S[0][1] = S[1][0] = s * (M[0][1] + M[1][0]);
+  S[0][2] = S[2][0] = s * (M[0][2] + M[2][0]);
+  S[1][2] = S[2][1] = s * (M[1][2] + M[2][1]);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * A[0][k] + S[1][l] * A[1][k] + S[2][l] * A[2][k];
+}
+
+// x-transpose accumulation of one quadrature point's dv into the row accumulators
+template <int qa, typename T>
+__device__ __forceinline__ void acc_row(const T dv[3][3], T P0[3][3], T P1[3][3], T P2[3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      if (nzD(qa, i)) P0[c][i] = fmac(gD(qa, i), dv[0][c], P0[c][i]);
+      if (nzB(qa, i)) P1[c][i] = fmac(gB(qa, i), dv[1][c], P1[c][i]);
+      if (nzB(qa, i)) P2[c][i] = fmac(gB(qa, i), dv[2][c], P2[c][i]);
+    }
+}
+
+// y-transpose of a finished row into the slab accumulators
+template <int qb, typename T>
+__device__ __forceinline__ void acc_slab(const T P0[3][3], const T P1[3][3], const T P2[3][3], T RB[3][3][3],
+                                         T RD[3][3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        if (nzB(qb, j)) RB[c][j][i] = fmac(gB(qb, j), P0[c][i], RB[c][j][i]);
+        if (nzD(qb, j)) RB[c][j][i] = fmac(gD(qb, j), P1[c][i], RB[c][j][i]);
+        if (nzB(qb, j)) RD[c][j][i] = fmac(gB(qb, j), P2[c][i], RD[c][j][i]);
+      }
+}
+
+// x- and y-transpose of one quadrature point's dv straight into the slab
+// accumulators (no per-row accumulator kept live):
+//   RB[c][j][i] += B(qb,j) D(qa,i) dv[0][c] + D(qb,j) B(qa,i) dv[1][c]
+//   RD[c][j][i] += B(qb,j) B(qa,i) dv[2][c]
+template <int qa, int qb, typename T>
+__device__ __forceinline__ void acc_direct(const T dv[3][3], T RB[3][3][3], T RD[3][3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        if (nzB(qb, j) && nzD(qa, i)) RB[c][j][i] = fmac(gB(qb, j) * gD(qa, i), dv[0][c], RB[c][j][i]);
+        if (nzD(qb, j) && nzB(qa, i)) RB[c][j][i] = fmac(gD(qb, j) * gB(qa, i), dv[1][c], RB[c][j][i]);
+        if (nzB(qb, j) && nzB(qa, i)) RD[c][j][i] = fmac(gB(qb, j) * gB(qa, i), dv[2][c], RD[c][j][i]);
+      }
+}
+
+// keep the scheduler from hoisting the next row's LDS reads into this one
+__device__ __forceinline__ void row_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// z-transpose of a finished slab into the element output Y[c][k][j][i]
+template <int qc, typename T>
+__device__ __forceinline__ void acc_elem(const T RB[3][3][3], const T RD[3][3][3], T Y[3][3][3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          if (nzB(qc, k)) Y[c][k][j][i] = fmac(gB(qc, k), RB[c][j][i], Y[c][k][j][i]);
+          if (nzD(qc, k)) Y[c][k][j][i] = fmac(gD(qc, k), RD[c][j][i], Y[c][k][j][i]);
+        }
+}
+
+template <typename T, int N>
+__device__ __forceinline__ void zero(T (&a)[N]) {
+#pragma unroll
+  for (int i = 0; i < N; i++) a[i] = T(0);
+}
+template <typename T>
+__device__ __forceinline__ void zero33(T a[3][3]) {
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) a[i][j] = T(0);
+}
+
+}  // namespace ptq2

@@ -1,0 +1,30 @@
+// ptq2_kernels.h -- host-side launchers of the operator kernels (C++ linkage).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ptq2 {
+struct Box;
+
+// error bits raised by validation kernels into a device int
+constexpr int ERRBIT_MESH = 1;
+constexpr int ERRBIT_VISCOSITY = 2;
+constexpr int ERRBIT_NONFINITE = 4;
+
+hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
+                           const double *eta_bm, double *y, bool add, hipStream_t s);
+hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
+                           hipStream_t s);
+hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, int *err,
+                              hipStream_t s);
+hipError_t launch_eta_to_bm(const Box &b, const double *eta, double *eta_bm, int *err, hipStream_t s);
+hipError_t launch_check_mesh(const Box &b, const double *coords, int *err, hipStream_t s);
+hipError_t launch_a21(const Box &b, const double *coords, const double *u, const uint8_t *mask, double *yp, bool add,
+                      hipStream_t s);
+hipError_t launch_a12(const Box &b, const double *coords, const double *p, double *y, bool add, hipStream_t s);
+hipError_t launch_diag(const Box &b, const double *coords, const double *eta_bm, double *d, bool add, hipStream_t s);
+template <typename T>
+hipError_t launch_bc_rows(int64_t n, const int64_t *idx, const T *x, T *y, int mode, hipStream_t s);
+template <typename T>
+hipError_t launch_scan_finite(int64_t n, const T *x, int *err, hipStream_t s);
+}  // namespace ptq2

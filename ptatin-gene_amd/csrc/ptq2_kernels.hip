@@ -1,0 +1,681 @@
+// ptq2_kernels.hip -- gfx950 kernels of the matrix-free Q2-P1disc Stokes operators.
+//
+// A11 (viscous block), A12 (gradient), A21 (divergence), the full block and the
+// A11 diagonal.  The element algorithm is the sum-factorized 3x3x3 tensor
+// contraction of src/stokes_operators_tensor.c:181-286 reorganised for one
+// element per lane (see ptq2_device.h for the brick/tile decomposition).
+#include "ptq2_device.h"
+#include "ptq2_kernels.h"
+
+namespace ptq2 {
+
+// ----------------------------------------------------------------- A11 FP64
+
+// geometry of one quadrature point of row (qb,qc): adjugate and scale
+// s = w eta / det, from the coordinate row data.  eta9 holds the slab's 9 values.
+template <int qa, int qb, int qc>
+__device__ __forceinline__ void geo_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                       const double *eta9, double A[3][3], double &s) {
+  double J[3][3];
+  qp_grad<qa>(xr0, xr1, xr2, J);
+  double det = adjugate(J, A);
+  constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
+  s = (w * eta9[qa + 3 * qb]) * rcp64(det);
+}
+
+template <int qb, int qc>
+__device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, int base, const double *eta9,
+                                          double RB[3][3][3], double RD[3][3][3]) {
+  double A0[3][3], A1[3][3], A2[3][3], s0, s1, s2;
+  {
+    double xr0[3][3], xr1[3][3], xr2[3][3];
+    row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+    geo_qp<0, qb, qc>(xr0, xr1, xr2, eta9, A0, s0);
+    geo_qp<1, qb, qc>(xr0, xr1, xr2, eta9, A1, s1);
+    geo_qp<2, qb, qc>(xr0, xr1, xr2, eta9, A2, s2);
+  }
+  double ur0[3][3], ur1[3][3], ur2[3][3];
+  row_data<qb, qc>(Us, base, ur0, ur1, ur2);
+  double g[3][3], dv[3][3];
+  qp_grad<0>(ur0, ur1, ur2, g); stress_action(A0, g, s0, dv); acc_direct<0, qb>(dv, RB, RD);
+  qp_grad<1>(ur0, ur1, ur2, g); stress_action(A1, g, s1, dv); acc_direct<1, qb>(dv, RB, RD);
+  qp_grad<2>(ur0, ur1, ur2, g); stress_action(A2, g, s2, dv); acc_direct<2, qb>(dv, RB, RD);
+  row_fence();
+}
+
+template <int qc>
+__device__ __forceinline__ void a11_slab64(const double *Xs, const double *Us, int base, const double *eta9,
+                                           double Y[3][3][3][3]) {
+  double RB[3][3][3], RD[3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0;
+  a11_row64<0, qc>(Xs, Us, base, eta9, RB, RD);
+  a11_row64<1, qc>(Xs, Us, base, eta9, RB, RD);
+  a11_row64<2, qc>(Xs, Us, base, eta9, RB, RD);
+  acc_elem<qc>(RB, RD, Y);
+  row_fence();
+}
+
+template <typename T>
+__device__ __forceinline__ void emit_element(T *Ys, int base, const T Y[3][3][3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) lds_add(&Ys[c * TCS + base + k * TPL + j * TROW + i], Y[c][k][j][i]);
+}
+
+__device__ __forceinline__ void load9(const double *__restrict__ ep, int qc, double *e9) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) e9[q] = ep[(9 * qc + q) * BEL];
+}
+
+__global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double *__restrict__ coords,
+                                                  const double *__restrict__ u, const uint8_t *__restrict__ mask,
+                                                  const double *__restrict__ eta_bm, double *__restrict__ y,
+                                                  int add_mode) {
+  __shared__ double Xs[3 * TCS];
+  __shared__ double Us[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_of(b, color, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
+  double ea[9], eb[9];
+  load9(ep, 0, ea);
+  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  load_tile(b, t, u, mask, Us, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  double Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
+  if (active) {
+    load9(ep, 1, eb);
+    a11_slab64<0>(Xs, Us, base, ea, Y);
+    load9(ep, 2, ea);
+    a11_slab64<1>(Xs, Us, base, eb, Y);
+    a11_slab64<2>(Xs, Us, base, ea, Y);
+  }
+  __syncthreads();
+  zero_tile(Xs, lane);
+  __syncthreads();
+  if (active) emit_element(Xs, base, Y);
+  __syncthreads();
+  store_tile(b, br, t, Xs, y, add_mode != 0, lane);
+}
+
+// ----------------------------------------------------------------- A11 FP32
+// Geometric factors are precomputed in FP64 (k_build_fac32) and stored rounded
+// to FP32 as F = sqrt(w eta / det) * adj(J), 9 per quadrature point, brick-major
+// [brick][q][m][lane] so that every load instruction is one contiguous 256-byte
+// segment.  Then dv = F^T (M + M^T) with M = F g (diagonal doubled), all FP32.
+
+template <int qa, int qb, int qc>
+__device__ __forceinline__ void a11_qp32(const float ur0[3][3], const float ur1[3][3], const float ur2[3][3],
+                                         const float *__restrict__ fp, float RB[3][3][3], float RD[3][3][3]) {
+  constexpr int q = qa + 3 * qb + 9 * qc;
+  float F[3][3];
+#pragma unroll
+  for (int m = 0; m < 9; m++) F[m / 3][m % 3] = fp[(q * 9 + m) * BEL];
+  float g[3][3], dv[3][3];
+  qp_grad<qa>(ur0, ur1, ur2, g);
+  float M[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) M[k][l] = g[0][l] * F[k][0] + g[1][l] * F[k][1] + g[2][l] * F[k][2];
+  float S[3][3];
+  S[0][0] = 2.0f * M[0][0];
+  S[1][1] = 2.0f * M[1][1];
+  S[2][2] = 2.0f * M[2][2];
+  S[0][1] = S[1][0] = M[0][1] + M[1][0];
+  S[0][2] = S[2][0] = M[0][2] + M[2][0];
+  S[1][2] = S[2][1] = M[1][2] + M[2][1];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * F[0][k] + S[1][l] * F[1][k] + S[2][l] * F[2][k];
+  acc_direct<qa, qb>(dv, RB, RD);
+}
+
+template <int qb, int qc>
+__device__ __forceinline__ void a11_row32(const float *Us, int base, const float *__restrict__ fp, float RB[3][3][3],
+                                          float RD[3][3][3]) {
+  float ur0[3][3], ur1[3][3], ur2[3][3];
+  row_data<qb, qc>(Us, base, ur0, ur1, ur2);
+  a11_qp32<0, qb, qc>(ur0, ur1, ur2, fp, RB, RD);
+  a11_qp32<1, qb, qc>(ur0, ur1, ur2, fp, RB, RD);
+  a11_qp32<2, qb, qc>(ur0, ur1, ur2, fp, RB, RD);
+  row_fence();
+}
+
+template <int qc>
+__device__ __forceinline__ void a11_slab32(const float *Us, int base, const float *__restrict__ fp,
+                                           float Y[3][3][3][3]) {
+  float RB[3][3][3], RD[3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0f;
+  a11_row32<0, qc>(Us, base, fp, RB, RD);
+  a11_row32<1, qc>(Us, base, fp, RB, RD);
+  a11_row32<2, qc>(Us, base, fp, RB, RD);
+  acc_elem<qc>(RB, RD, Y);
+  row_fence();
+}
+
+__global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
+                                                  const uint8_t *__restrict__ mask, const float *__restrict__ fac,
+                                                  float *__restrict__ y, int add_mode) {
+  __shared__ float Us[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_of(b, color, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, u, mask, Us, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  const float *fp = fac + (int64_t)br.lin * 243 * BEL + lane;
+  float Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
+  if (active) {
+    a11_slab32<0>(Us, base, fp, Y);
+    a11_slab32<1>(Us, base, fp, Y);
+    a11_slab32<2>(Us, base, fp, Y);
+  }
+  __syncthreads();
+  zero_tile(Us, lane);
+  __syncthreads();
+  if (active) emit_element(Us, base, Y);
+  __syncthreads();
+  store_tile(b, br, t, Us, y, add_mode != 0, lane);
+}
+
+// FP64 geometry -> FP32 factors, with validation of det > 0 and eta > 0.
+template <int qa, int qb, int qc>
+__device__ __forceinline__ void fac_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                       const double *eta27, float *__restrict__ fp, int &bad) {
+  double J[3][3], A[3][3];
+  qp_grad<qa>(xr0, xr1, xr2, J);
+  double det = adjugate(J, A);
+  constexpr int q = qa + 3 * qb + 9 * qc;
+  constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
+  double et = eta27[q];
+  if (!(det > 0.0) || !isfinite(det)) bad |= 1;
+  if (!(et > 0.0) || !isfinite(et)) bad |= 2;
+  double s = (bad == 0) ? sqrt(w * et / det) : 0.0;
+#pragma unroll
+  for (int m = 0; m < 9; m++) fp[(q * 9 + m) * BEL] = (float)(s * A[m / 3][m % 3]);
+}
+
+template <int qb, int qc>
+__device__ __forceinline__ void fac_row(const double *Xs, int base, const double *eta27, float *fp, int &bad) {
+  double xr0[3][3], xr1[3][3], xr2[3][3];
+  row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+  fac_qp<0, qb, qc>(xr0, xr1, xr2, eta27, fp, bad);
+  fac_qp<1, qb, qc>(xr0, xr1, xr2, eta27, fp, bad);
+  fac_qp<2, qb, qc>(xr0, xr1, xr2, eta27, fp, bad);
+  row_fence();
+}
+
+__global__ __launch_bounds__(BEL) void k_build_fac32(Box b, const double *__restrict__ coords,
+                                                     const double *__restrict__ eta_bm, float *__restrict__ fac,
+                                                     int *__restrict__ err) {
+  __shared__ double Xs[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_lin(b, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  __syncthreads();
+  if (!lane_active(t, lane)) return;
+  const int base = lane_base(lane);
+  double eta27[27];
+  const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
+#pragma unroll
+  for (int q = 0; q < 27; q++) eta27[q] = ep[q * BEL];
+  float *fp = fac + (int64_t)br.lin * 243 * BEL + lane;
+  int bad = 0;
+  fac_row<0, 0>(Xs, base, eta27, fp, bad); fac_row<1, 0>(Xs, base, eta27, fp, bad); fac_row<2, 0>(Xs, base, eta27, fp, bad);
+  fac_row<0, 1>(Xs, base, eta27, fp, bad); fac_row<1, 1>(Xs, base, eta27, fp, bad); fac_row<2, 1>(Xs, base, eta27, fp, bad);
+  fac_row<0, 2>(Xs, base, eta27, fp, bad); fac_row<1, 2>(Xs, base, eta27, fp, bad); fac_row<2, 2>(Xs, base, eta27, fp, bad);
+  if (bad & 1) atomicOr(err, ERRBIT_MESH);
+  if (bad & 2) atomicOr(err, ERRBIT_VISCOSITY);
+}
+
+// Natural eta[27e+q] -> brick-major [brick][q][lane]; validates eta > 0 and finite.
+__global__ void k_eta_to_bm(Box b, const double *__restrict__ eta, double *__restrict__ eta_bm,
+                            int *__restrict__ err) {
+  const int lane = threadIdx.x & (BEL - 1);
+  const int64_t blin = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / BEL;
+  const int64_t nb = (int64_t)b.nbx * b.nby * b.nbz;
+  if (blin >= nb) return;
+  BrickId br = brick_lin(b, (int)blin);
+  Tile t = tile_of(b, br);
+  double *dst = eta_bm + blin * 27 * BEL + lane;
+  if (!lane_active(t, lane)) {
+    for (int q = 0; q < 27; q++) dst[q * BEL] = 1.0;
+    return;
+  }
+  int64_t e = lane_element(b, t, lane);
+  int bad = 0;
+  for (int q = 0; q < 27; q++) {
+    double v = eta[27 * e + q];
+    if (!(v > 0.0) || !isfinite(v)) bad = 1;
+    dst[q * BEL] = v;
+  }
+  if (bad) atomicOr(err, ERRBIT_VISCOSITY);
+}
+
+// Mesh validation: det J > 0 and finite at every quadrature point.
+template <int qb, int qc>
+__device__ __forceinline__ void check_row(const double *Xs, int base, int &bad) {
+  double xr0[3][3], xr1[3][3], xr2[3][3], J[3][3], A[3][3], det;
+  row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+  qp_grad<0>(xr0, xr1, xr2, J); det = adjugate(J, A); if (!(det > 0.0) || !isfinite(det)) bad = 1;
+  qp_grad<1>(xr0, xr1, xr2, J); det = adjugate(J, A); if (!(det > 0.0) || !isfinite(det)) bad = 1;
+  qp_grad<2>(xr0, xr1, xr2, J); det = adjugate(J, A); if (!(det > 0.0) || !isfinite(det)) bad = 1;
+  row_fence();
+}
+
+__global__ __launch_bounds__(BEL) void k_check_mesh(Box b, const double *__restrict__ coords, int *__restrict__ err) {
+  __shared__ double Xs[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_lin(b, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  __syncthreads();
+  if (!lane_active(t, lane)) return;
+  const int base = lane_base(lane);
+  int bad = 0;
+  check_row<0, 0>(Xs, base, bad); check_row<1, 0>(Xs, base, bad); check_row<2, 0>(Xs, base, bad);
+  check_row<0, 1>(Xs, base, bad); check_row<1, 1>(Xs, base, bad); check_row<2, 1>(Xs, base, bad);
+  check_row<0, 2>(Xs, base, bad); check_row<1, 2>(Xs, base, bad); check_row<2, 2>(Xs, base, bad);
+  if (bad) atomicOr(err, ERRBIT_MESH);
+}
+
+// ------------------------------------------------------- pressure basis helper
+// P3D_ConstructNi_P1GRel_3D (src/element_utils_q2.c:183-224):
+// N^p = [1, (x-xbar)/Lx, (y-ybar)/Ly, (z-zbar)/Lz], xbar the mean of the 27 nodes,
+// Lx = x_14 - x_12, Ly = y_16 - y_10, Lz = z_22 - z_4.
+struct PBasis {
+  double avg[3], inv[3];
+};
+__device__ __forceinline__ PBasis pbasis_setup(const double *Xs, int base) {
+  PBasis pb;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) s += tv(Xs, base, c, i, j, k);
+    pb.avg[c] = s * (1.0 / 27.0);
+  }
+  pb.inv[0] = 1.0 / (tv(Xs, base, 0, 2, 1, 1) - tv(Xs, base, 0, 0, 1, 1));
+  pb.inv[1] = 1.0 / (tv(Xs, base, 1, 1, 2, 1) - tv(Xs, base, 1, 1, 0, 1));
+  pb.inv[2] = 1.0 / (tv(Xs, base, 2, 1, 1, 2) - tv(Xs, base, 2, 1, 1, 0));
+  return pb;
+}
+
+// ------------------------------------------------------------------ A21
+// y_p[4e+i] += -sum_q w det N^p_i (div u)_q ; div u = trace(M)/det.
+
+template <int qa, int qb, int qc>
+__device__ __forceinline__ void a21_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                       const double ur0[3][3], const double ur1[3][3], const double ur2[3][3],
+                                       const PBasis &pb, double Yp[4]) {
+  double J[3][3], A[3][3], g[3][3], x[3];
+  qp_grad<qa>(xr0, xr1, xr2, J);
+  adjugate(J, A);
+  qp_grad<qa>(ur0, ur1, ur2, g);
+  qp_value<qa>(xr0, x);
+  // trace(M) = sum_k sum_m g[m][k] A[k][m] = det * div u
+  double tr = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) tr += g[0][k] * A[k][0] + g[1][k] * A[k][1] + g[2][k] * A[k][2];
+  constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
+  double f = -w * tr;
+  Yp[0] += f;
+#pragma unroll
+  for (int d = 0; d < 3; d++) Yp[1 + d] += f * ((x[d] - pb.avg[d]) * pb.inv[d]);
+}
+
+template <int qb, int qc>
+__device__ __forceinline__ void a21_row(const double *Xs, const double *Us, int base, const PBasis &pb, double Yp[4]) {
+  double xr0[3][3], xr1[3][3], xr2[3][3], ur0[3][3], ur1[3][3], ur2[3][3];
+  row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+  row_data<qb, qc>(Us, base, ur0, ur1, ur2);
+  a21_qp<0, qb, qc>(xr0, xr1, xr2, ur0, ur1, ur2, pb, Yp);
+  a21_qp<1, qb, qc>(xr0, xr1, xr2, ur0, ur1, ur2, pb, Yp);
+  a21_qp<2, qb, qc>(xr0, xr1, xr2, ur0, ur1, ur2, pb, Yp);
+  row_fence();
+}
+
+__global__ __launch_bounds__(BEL) void k_a21(Box b, const double *__restrict__ coords, const double *__restrict__ u,
+                                             const uint8_t *__restrict__ mask, double *__restrict__ yp, int add_mode) {
+  __shared__ double Xs[3 * TCS];
+  __shared__ double Us[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_lin(b, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  load_tile(b, t, u, mask, Us, lane);
+  __syncthreads();
+  if (!lane_active(t, lane)) return;
+  const int base = lane_base(lane);
+  const PBasis pb = pbasis_setup(Xs, base);
+  double Yp[4] = {0.0, 0.0, 0.0, 0.0};
+  a21_row<0, 0>(Xs, Us, base, pb, Yp); a21_row<1, 0>(Xs, Us, base, pb, Yp); a21_row<2, 0>(Xs, Us, base, pb, Yp);
+  a21_row<0, 1>(Xs, Us, base, pb, Yp); a21_row<1, 1>(Xs, Us, base, pb, Yp); a21_row<2, 1>(Xs, Us, base, pb, Yp);
+  a21_row<0, 2>(Xs, Us, base, pb, Yp); a21_row<1, 2>(Xs, Us, base, pb, Yp); a21_row<2, 2>(Xs, Us, base, pb, Yp);
+  const int64_t e = lane_element(b, t, lane);
+  double *o = yp + 4 * e;
+#pragma unroll
+  for (int i = 0; i < 4; i++) o[i] = add_mode ? o[i] + Yp[i] : Yp[i];
+}
+
+// ------------------------------------------------------------------ A12
+// y_u[n,l] += -sum_q w p_q (A^T)... : dv[k][l] = -w p A[l][k] (QuadratureAction_A12_AVX).
+
+template <int qa, int qb, int qc>
+__device__ __forceinline__ void a12_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                       const PBasis &pb, const double pe[4], double RB[3][3][3], double RD[3][3][3]) {
+  double J[3][3], A[3][3], x[3], dv[3][3];
+  qp_grad<qa>(xr0, xr1, xr2, J);
+  adjugate(J, A);
+  qp_value<qa>(xr0, x);
+  double pq = pe[0];
+#pragma unroll
+  for (int d = 0; d < 3; d++) pq += pe[1 + d] * ((x[d] - pb.avg[d]) * pb.inv[d]);
+  constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
+  double f = -w * pq;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) dv[k][l] = f * A[l][k];
+  acc_direct<qa, qb>(dv, RB, RD);
+}
+
+template <int qb, int qc>
+__device__ __forceinline__ void a12_row(const double *Xs, int base, const PBasis &pb, const double pe[4],
+                                        double RB[3][3][3], double RD[3][3][3]) {
+  double xr0[3][3], xr1[3][3], xr2[3][3];
+  row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+  a12_qp<0, qb, qc>(xr0, xr1, xr2, pb, pe, RB, RD);
+  a12_qp<1, qb, qc>(xr0, xr1, xr2, pb, pe, RB, RD);
+  a12_qp<2, qb, qc>(xr0, xr1, xr2, pb, pe, RB, RD);
+  row_fence();
+}
+
+template <int qc>
+__device__ __forceinline__ void a12_slab(const double *Xs, int base, const PBasis &pb, const double pe[4],
+                                         double Y[3][3][3][3]) {
+  double RB[3][3][3], RD[3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0;
+  a12_row<0, qc>(Xs, base, pb, pe, RB, RD);
+  a12_row<1, qc>(Xs, base, pb, pe, RB, RD);
+  a12_row<2, qc>(Xs, base, pb, pe, RB, RD);
+  acc_elem<qc>(RB, RD, Y);
+  row_fence();
+}
+
+__global__ __launch_bounds__(BEL) void k_a12(Box b, int color, const double *__restrict__ coords,
+                                             const double *__restrict__ p, double *__restrict__ y, int add_mode) {
+  __shared__ double Xs[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_of(b, color, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  double Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
+  if (active) {
+    const PBasis pb = pbasis_setup(Xs, base);
+    const int64_t e = lane_element(b, t, lane);
+    double pe[4] = {p[4 * e], p[4 * e + 1], p[4 * e + 2], p[4 * e + 3]};
+    a12_slab<0>(Xs, base, pb, pe, Y);
+    a12_slab<1>(Xs, base, pb, pe, Y);
+    a12_slab<2>(Xs, base, pb, pe, Y);
+  }
+  __syncthreads();
+  zero_tile(Xs, lane);
+  __syncthreads();
+  if (active) emit_element(Xs, base, Y);
+  __syncthreads();
+  store_tile(b, br, t, Xs, y, add_mode != 0, lane);
+}
+
+// ---------------------------------------------------------------- diagonal
+// diag(n,x) += sum_q w det eta (2 Nx^2 + Ny^2 + Nz^2) (cyclic for y, z), the form of
+// MatMultMF_Stokes_MixedFEM3d_diagB11 (src/stokes_q2p1_mf_operators_diag_def.c:120-205).
+// With h = A grad_xi N = det grad_x N: w det eta Nx^2 = (w eta / det) h_x^2.
+
+template <int qa, int qb, int qc>
+__device__ __forceinline__ void diag_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                        const double *eta9, double Y[3][3][3][3]) {
+  double J[3][3], A[3][3];
+  qp_grad<qa>(xr0, xr1, xr2, J);
+  double det = adjugate(J, A);
+  constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
+  double s = (w * eta9[qa + 3 * qb]) * rcp64(det);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const double n0 = gD(qa, i) * gB(qb, j) * gB(qc, k);
+        const double n1 = gB(qa, i) * gD(qb, j) * gB(qc, k);
+        const double n2 = gB(qa, i) * gB(qb, j) * gD(qc, k);
+        if (n0 != 0.0 || n1 != 0.0 || n2 != 0.0) {
+          double hx = A[0][0] * n0 + A[0][1] * n1 + A[0][2] * n2;
+          double hy = A[1][0] * n0 + A[1][1] * n1 + A[1][2] * n2;
+          double hz = A[2][0] * n0 + A[2][1] * n1 + A[2][2] * n2;
+          double ax = hx * hx, ay = hy * hy, az = hz * hz, sum = ax + ay + az;
+          Y[0][k][j][i] += s * (sum + ax);
+          Y[1][k][j][i] += s * (sum + ay);
+          Y[2][k][j][i] += s * (sum + az);
+        }
+      }
+}
+
+template <int qb, int qc>
+__device__ __forceinline__ void diag_row(const double *Xs, int base, const double *eta9, double Y[3][3][3][3]) {
+  double xr0[3][3], xr1[3][3], xr2[3][3];
+  row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+  diag_qp<0, qb, qc>(xr0, xr1, xr2, eta9, Y);
+  row_fence();
+  diag_qp<1, qb, qc>(xr0, xr1, xr2, eta9, Y);
+  row_fence();
+  diag_qp<2, qb, qc>(xr0, xr1, xr2, eta9, Y);
+  row_fence();
+}
+
+__global__ __launch_bounds__(BEL) void k_diag(Box b, int color, const double *__restrict__ coords,
+                                              const double *__restrict__ eta_bm, double *__restrict__ d, int add_mode) {
+  __shared__ double Xs[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_of(b, color, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  double Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
+  if (active) {
+    const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
+    double e9[9];
+    load9(ep, 0, e9);
+    diag_row<0, 0>(Xs, base, e9, Y); diag_row<1, 0>(Xs, base, e9, Y); diag_row<2, 0>(Xs, base, e9, Y);
+    load9(ep, 1, e9);
+    diag_row<0, 1>(Xs, base, e9, Y); diag_row<1, 1>(Xs, base, e9, Y); diag_row<2, 1>(Xs, base, e9, Y);
+    load9(ep, 2, e9);
+    diag_row<0, 2>(Xs, base, e9, Y); diag_row<1, 2>(Xs, base, e9, Y); diag_row<2, 2>(Xs, base, e9, Y);
+  }
+  __syncthreads();
+  zero_tile(Xs, lane);
+  __syncthreads();
+  if (active) emit_element(Xs, base, Y);
+  __syncthreads();
+  store_tile(b, br, t, Xs, d, add_mode != 0, lane);
+}
+
+// ------------------------------------------------------------- BC helpers
+// Dirichlet rows: y[idx] = x[idx] (mode 0, BCListInsertDirichlet_MatMult), 0 (mode 1,
+// BCListInsertZero) or 1 (mode 2, BCListInsertValueIntoDirichletSlot).
+template <typename T>
+__global__ void k_bc_rows(int64_t n, const int64_t *__restrict__ idx, const T *__restrict__ x, T *__restrict__ y,
+                          int mode) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t k = idx[i];
+  y[k] = mode == 0 ? x[k] : (mode == 1 ? T(0) : T(1));
+}
+
+// Non-finite scan of an input vector (sets ERRBIT_NONFINITE).
+template <typename T>
+__global__ void k_scan_finite(int64_t n, const T *__restrict__ x, int *__restrict__ err) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int bad = 0;
+  for (; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    if (!isfinite(x[i])) bad = 1;
+  if (bad) atomicOr(err, ERRBIT_NONFINITE);
+}
+
+// ------------------------------------------------------------ launchers
+static inline int nbricks_color(const Box &b, int color) {
+  int cx = color & 1, cy = (color >> 1) & 1, cz = (color >> 2) & 1;
+  return ((b.nbx - cx + 1) >> 1) * ((b.nby - cy + 1) >> 1) * ((b.nbz - cz + 1) >> 1);
+}
+static inline int nbricks(const Box &b) { return b.nbx * b.nby * b.nbz; }
+
+hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
+                           const double *eta_bm, double *y, bool add, hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_a11_fp64, dim3(nb), dim3(BEL), 0, s, b, c, coords, u, mask, eta_bm, y, add ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
+                           hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_a11_fp32, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, fac, y, add ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, int *err,
+                              hipStream_t s) {
+  hipLaunchKernelGGL(k_build_fac32, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, eta_bm, fac, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_eta_to_bm(const Box &b, const double *eta, double *eta_bm, int *err, hipStream_t s) {
+  int64_t threads = (int64_t)nbricks(b) * BEL;
+  int blocks = (int)((threads + 255) / 256);
+  hipLaunchKernelGGL(k_eta_to_bm, dim3(blocks), dim3(256), 0, s, b, eta, eta_bm, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_check_mesh(const Box &b, const double *coords, int *err, hipStream_t s) {
+  hipLaunchKernelGGL(k_check_mesh, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_a21(const Box &b, const double *coords, const double *u, const uint8_t *mask, double *yp, bool add,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_a21, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, u, mask, yp, add ? 1 : 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_a12(const Box &b, const double *coords, const double *p, double *y, bool add, hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_a12, dim3(nb), dim3(BEL), 0, s, b, c, coords, p, y, add ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_diag(const Box &b, const double *coords, const double *eta_bm, double *d, bool add, hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_diag, dim3(nb), dim3(BEL), 0, s, b, c, coords, eta_bm, d, add ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_bc_rows(int64_t n, const int64_t *idx, const T *x, T *y, int mode, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bc_rows<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, idx, x, y, mode);
+  return hipGetLastError();
+}
+template hipError_t launch_bc_rows<double>(int64_t, const int64_t *, const double *, double *, int, hipStream_t);
+template hipError_t launch_bc_rows<float>(int64_t, const int64_t *, const float *, float *, int, hipStream_t);
+
+template <typename T>
+hipError_t launch_scan_finite(int64_t n, const T *x, int *err, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_scan_finite<T>, dim3((unsigned)blocks), dim3(256), 0, s, n, x, err);
+  return hipGetLastError();
+}
+template hipError_t launch_scan_finite<double>(int64_t, const double *, int *, hipStream_t);
+template hipError_t launch_scan_finite<float>(int64_t, const float *, int *, hipStream_t);
+
+}  // namespace ptq2

@@ -1,0 +1,244 @@
+"""Host-side mirror of pTatin3D's matrix-free Stokes operator objects, backed by libptq2.
+
+Names, argument meaning and error behaviour follow the reference:
+  MatA11MF / MatA11MFCreate / MatA11MFSetup / MatA11MFDestroy   (src/stokes_operators.c:104-340)
+  MatMult_MFStokes_A11, MatGetDiagonal_MFStokes_A11             (src/stokes_operators.c:1072-1222)
+  MatStokesMF with MatMult_MFStokes_A12 / _A21 / _A             (src/stokes_operators.c:893-1509)
+  the SpMVOp_MatMult seam (include/stokes_operators.h:48-64)   -> MatA11MF.spmv
+The backend registry mirrors `-a11_op` (src/stokes_operators.c:123-154): unknown names raise.
+
+Vectors are numpy arrays (staged through the device, synchronous) or torch tensors on
+the GPU (device pointers, ordered on torch's current stream).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import ADD, DEVICE, FP32, MATMULT, check
+
+try:  # torch is the device-memory and stream plumbing of the device path
+    import torch
+except ImportError:  # pragma: no cover
+    torch = None
+
+
+def _is_dev(x):
+    return torch is not None and isinstance(x, torch.Tensor) and x.is_cuda
+
+
+class Context:
+    """One libptq2 context: a device-resident mesh plus quadrature-point viscosity."""
+
+    def __init__(self, device=0):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        check(L.ptq2_create(ctypes.byref(h), int(device)), "ptq2_create")
+        self._h = h
+        self.device = device
+        self.mx = self.my = self.mz = None
+        self.nvel = self.npres = self.nel = 0
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("context destroyed")
+        return self._h
+
+    def destroy(self):
+        if getattr(self, "_h", None) is not None:
+            _lib.lib().ptq2_destroy(ctypes.byref(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _stream(self, *tensors):
+        if any(_is_dev(t) for t in tensors):
+            s = torch.cuda.current_stream(device=tensors[0].device).cuda_stream
+            _lib.lib().ptq2_set_stream(self.handle, ctypes.c_void_p(s))
+
+    def setup_mesh(self, mx, my, mz, coords, mask=None, elnidx=None):
+        coords = np.ascontiguousarray(coords, dtype=np.float64)
+        mptr = None
+        if mask is not None:
+            mask = np.ascontiguousarray(mask, dtype=np.uint8)
+            mptr = mask.ctypes.data
+        eptr = None
+        if elnidx is not None:
+            elnidx = np.ascontiguousarray(elnidx, dtype=np.int32)
+            eptr = elnidx.ctypes.data
+        check(_lib.lib().ptq2_setup_mesh(self.handle, mx, my, mz, coords.ctypes.data, eptr, mptr),
+              "ptq2_setup_mesh", self.handle)
+        self.mx, self.my, self.mz = mx, my, mz
+        nv, np_, ne = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(_lib.lib().ptq2_sizes(self.handle, ctypes.byref(nv), ctypes.byref(np_), ctypes.byref(ne)),
+              "ptq2_sizes", self.handle)
+        self.nvel, self.npres, self.nel = nv.value, np_.value, ne.value
+
+    def set_viscosity(self, eta, state=0):
+        if _is_dev(eta):
+            self._stream(eta)
+            check(_lib.lib().ptq2_set_viscosity(self.handle, eta.data_ptr(), state, DEVICE),
+                  "ptq2_set_viscosity", self.handle)
+        else:
+            eta = np.ascontiguousarray(eta, dtype=np.float64)
+            check(_lib.lib().ptq2_set_viscosity(self.handle, eta.ctypes.data, state, 0),
+                  "ptq2_set_viscosity", self.handle)
+
+    def sync(self):
+        check(_lib.lib().ptq2_sync(self.handle), "ptq2_sync", self.handle)
+
+    # generic dispatch of one-input/one-output entry points
+    def _call1(self, fn, name, x, y, flags, xdt, ydt, ny):
+        if _is_dev(x):
+            if y is None:
+                y = torch.zeros(ny, dtype=ydt, device=x.device)
+            self._stream(x, y)
+            check(fn(self.handle, x.data_ptr(), y.data_ptr(), flags | DEVICE), name, self.handle)
+            return y
+        xn = np.ascontiguousarray(x, dtype=xdt)
+        if y is None:
+            y = np.zeros(ny, dtype=ydt)
+        check(fn(self.handle, xn.ctypes.data, y.ctypes.data, flags), name, self.handle)
+        return y
+
+    def a11(self, u, y=None, flags=0):
+        fp32 = bool(flags & FP32)
+        L = _lib.lib()
+        if _is_dev(u):
+            dt = torch.float32 if fp32 else torch.float64
+            return self._call1(L.ptq2_a11_apply, "ptq2_a11_apply", u, y, flags, None, dt, self.nvel)
+        dt = np.float32 if fp32 else np.float64
+        return self._call1(L.ptq2_a11_apply, "ptq2_a11_apply", u, y, flags, dt, dt, self.nvel)
+
+    def a11_diag(self, d=None, flags=0, device=None):
+        L = _lib.lib()
+        if device is not None or _is_dev(d):
+            if d is None:
+                d = torch.zeros(self.nvel, dtype=torch.float64, device=device)
+            self._stream(d)
+            check(L.ptq2_a11_diag(self.handle, d.data_ptr(), flags | DEVICE), "ptq2_a11_diag", self.handle)
+            return d
+        if d is None:
+            d = np.zeros(self.nvel)
+        check(L.ptq2_a11_diag(self.handle, d.ctypes.data, flags), "ptq2_a11_diag", self.handle)
+        return d
+
+    def a12(self, p, y=None, flags=0):
+        L = _lib.lib()
+        dt = torch.float64 if _is_dev(p) else np.float64
+        return self._call1(L.ptq2_a12_apply, "ptq2_a12_apply", p, y, flags, np.float64, dt, self.nvel)
+
+    def a21(self, u, y=None, flags=0):
+        L = _lib.lib()
+        dt = torch.float64 if _is_dev(u) else np.float64
+        return self._call1(L.ptq2_a21_apply, "ptq2_a21_apply", u, y, flags, np.float64, dt, self.npres)
+
+    def stokes(self, u, p, yu=None, yp=None, flags=0):
+        L = _lib.lib()
+        if _is_dev(u):
+            if yu is None:
+                yu = torch.zeros(self.nvel, dtype=torch.float64, device=u.device)
+            if yp is None:
+                yp = torch.zeros(self.npres, dtype=torch.float64, device=u.device)
+            self._stream(u, p, yu, yp)
+            check(L.ptq2_stokes_apply(self.handle, u.data_ptr(), p.data_ptr(), yu.data_ptr(), yp.data_ptr(),
+                                      flags | DEVICE), "ptq2_stokes_apply", self.handle)
+            return yu, yp
+        un = np.ascontiguousarray(u, dtype=np.float64)
+        pn = np.ascontiguousarray(p, dtype=np.float64)
+        if yu is None:
+            yu = np.zeros(self.nvel)
+        if yp is None:
+            yp = np.zeros(self.npres)
+        check(L.ptq2_stokes_apply(self.handle, un.ctypes.data, pn.ctypes.data, yu.ctypes.data, yp.ctypes.data,
+                                  flags), "ptq2_stokes_apply", self.handle)
+        return yu, yp
+
+
+# -------------------------------------------------------------- -a11_op registry
+_A11_OPS = {}
+
+
+def register_a11_op(name, factory):
+    """PetscFunctionListAdd analogue for A11 backends (src/stokes_operators.c:123-142)."""
+    _A11_OPS[name] = factory
+
+
+class MatA11MF:
+    """The A11 shell context (struct _p_MatA11MF, include/stokes_operators.h:48-64) with the
+    HIP backend behind SpMVOp_SetUp / SpMVOp_MatMult / SpMVOp_Destroy."""
+
+    def __init__(self, op="hip", device=0):
+        if op not in _A11_OPS:
+            raise ValueError("Unknown -a11_op %r (registered: %s)" % (op, ", ".join(sorted(_A11_OPS))))
+        self.op = op
+        self.ctx = _A11_OPS[op](device)
+        self.is_setup = False
+        self.state = 0
+
+    def setup(self, problem, state=0):
+        """MatA11MFSetup: mesh, BC list and quadrature-point viscosity."""
+        self.ctx.setup_mesh(problem.mx, problem.my, problem.mz, problem.coords, problem.mask)
+        self.ctx.set_viscosity(problem.eta, state)
+        self.state = state
+        self.is_setup = True
+
+    def set_viscosity(self, eta, state):
+        self.ctx.set_viscosity(eta, state)
+        self.state = state
+
+    def spmv(self, u, y, fp32=False):
+        """SpMVOp_MatMult contract: Yu += element sum of A11 u (no boundary handling)."""
+        self._need_setup()
+        return self.ctx.a11(u, y, ADD | (FP32 if fp32 else 0))
+
+    def mult(self, x, y=None, fp32=False):
+        """MatMult_MFStokes_A11: zero Dirichlet columns, apply, y_k = x_k on Dirichlet rows."""
+        self._need_setup()
+        return self.ctx.a11(x, y, MATMULT | (FP32 if fp32 else 0))
+
+    def get_diagonal(self, d=None, device=None):
+        """MatGetDiagonal_MFStokes_A11: diagonal with 1 in Dirichlet slots."""
+        self._need_setup()
+        return self.ctx.a11_diag(d, MATMULT, device)
+
+    def destroy(self):
+        self.ctx.destroy()
+        self.is_setup = False
+
+    def _need_setup(self):
+        if not self.is_setup:
+            raise RuntimeError("MF operator not setup")
+
+
+class MatStokesMF:
+    """Shell context of the Stokes blocks (A12, A21, full A)."""
+
+    def __init__(self, device=0):
+        self.ctx = Context(device)
+        self.is_setup = False
+
+    def setup(self, problem, state=0):
+        self.ctx.setup_mesh(problem.mx, problem.my, problem.mz, problem.coords, problem.mask)
+        self.ctx.set_viscosity(problem.eta, state)
+        self.is_setup = True
+
+    def mult_a12(self, p, y=None):
+        """MatMult_MFStokes_A12: y = A12 p with Dirichlet rows of y set to 0."""
+        return self.ctx.a12(p, y, MATMULT)
+
+    def mult_a21(self, u, y=None):
+        """MatMult_MFStokes_A21: y = A21 u with Dirichlet columns of u zeroed."""
+        return self.ctx.a21(u, y, MATMULT)
+
+    def mult_a(self, u, p, yu=None, yp=None):
+        """MatMult_MFStokes_A: [A11 A12; A21 0][u; p] with the velocity BCs."""
+        return self.ctx.stokes(u, p, yu, yp, MATMULT)
+
+
+register_a11_op("hip", lambda device: Context(device))

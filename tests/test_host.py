@@ -1,0 +1,100 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every entry
+point of include/ptq2.h, its error paths return codes (never abort), and the host-side
+mesh/BC/test-vector generators agree with the oracle's restatement."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+import ptq2
+from ptq2 import _lib
+from ptq2 import mesh as M
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(REPO, "include", "ptq2.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(ptq2_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_header_symbol():
+    L = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 15
+    for n in names:
+        assert hasattr(L, n), n
+    # the ctypes table covers exactly the header
+    assert sorted(_lib.SIGNATURES) == names
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (ptq2_\w+)", out))
+    assert set(names) <= exported
+
+
+def test_strerror_and_null_arguments():
+    L = _lib.lib()
+    for code in range(0, 10):
+        assert L.ptq2_strerror(code)
+    assert L.ptq2_create(None, 0) == _lib.ERR_INVALID_ARG
+    assert L.ptq2_destroy(None) == _lib.OK
+    assert L.ptq2_sync(None) == _lib.ERR_INVALID_ARG
+    assert L.ptq2_a11_apply(None, None, None, 0) == _lib.ERR_INVALID_ARG
+    assert L.ptq2_set_viscosity(None, None, 0, 0) == _lib.ERR_INVALID_ARG
+    assert L.ptq2_setup_mesh(None, 1, 1, 1, None, None, None) == _lib.ERR_INVALID_ARG
+
+
+def test_create_without_gpu_returns_code():
+    """On a host without a GPU (this container) creating a context reports an error code."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    h = ctypes.c_void_p()
+    rc = _lib.lib().ptq2_create(ctypes.byref(h), 0)
+    assert rc == _lib.ERR_DEVICE
+    with pytest.raises(ptq2.PTQ2Error):
+        ptq2.Context(0)
+
+
+def test_unknown_a11_op_raises():
+    with pytest.raises(ValueError):
+        ptq2.MatA11MF("avx")
+
+
+def test_mesh_generators_match_oracle():
+    for mx, my, mz in [(8, 9, 11), (1, 1, 3), (5, 7, 14)]:
+        c = M.q2_uniform(mx, my, mz)
+        assert np.array_equal(c, O.mesh_uniform(mx, my, mz))
+        b = M.bump_ymax(mx, my, mz, c)
+        assert np.array_equal(b, O.mesh_bump(mx, my, mz, c))
+        for kind, k in (("freeslip_freesurface", 0), ("freeslip", 1), ("noslip", 2)):
+            assert np.array_equal(M.bc_mask(mx, my, mz, kind), O.bc_mask(mx, my, mz, k))
+        assert np.array_equal(M.test_vector_u(mx, my, mz, b), O.test_vector_u(mx, my, mz, b))
+        with np.errstate(all="ignore"):
+            assert np.allclose(M.test_vector_p(mx, my, mz), O.test_vector_p(mx, my, mz), equal_nan=True)
+        assert np.array_equal(M.test_vector_p(mx, my, mz, legacy=False), O.test_vector_p(mx, my, mz, legacy=False))
+
+
+def test_splitmix_reference_values():
+    # first outputs of SplitMix64 with seed 0 (published test vector of the generator)
+    z = M.splitmix64(0, 3)
+    assert [int(v) for v in z] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    u = M.uniform(7, 10000)
+    assert 0.0 <= u.min() and u.max() < 1.0 and abs(u.mean() - 0.5) < 0.02
+
+
+def test_qp_coords_reproduce_affine_map():
+    m = 3
+    A = M.SHEAR
+    c = M.affine(M.q2_uniform(m, m, m), A, [0.5, 0.0, 1.0])
+    xq = M.qp_coords(m, m, m, c)
+    # on an affine mesh the QP of element (ex,ey,ez) sits at A((2e+1+xi)/(2m)) + b
+    ref = M.affine(M.qp_coords(m, m, m, M.q2_uniform(m, m, m)).reshape(-1), A, [0.5, 0.0, 1.0])
+    assert np.abs(xq.reshape(-1) - ref).max() < 1e-14
+    g = M.GAUSS_X
+    assert np.allclose(M.qp_coords(m, m, m, M.q2_uniform(m, m, m))[0, 0], (1 + g[0]) / (2 * m))
