@@ -57,8 +57,10 @@ int ptq2_destroy(ptq2_ctx **ctx);
 const char *ptq2_strerror(int code);
 /* message of the last failing call on this context ("" when none) */
 const char *ptq2_last_error(const ptq2_ctx *ctx);
-/* run subsequent work on a caller-owned hipStream_t (NULL: the context's own) */
+/* run subsequent work on the hipStream_t `hip_stream` (NULL is the null stream, as used by
+   a caller's default stream); a new context runs on a stream of its own */
 int ptq2_set_stream(ptq2_ctx *ctx, void *hip_stream);
+void *ptq2_get_stream(const ptq2_ctx *ctx);
 int ptq2_sync(ptq2_ctx *ctx);
 
 /* mesh and coefficients ---------------------------------------------------- */

@@ -1,0 +1,74 @@
+/*
+ * ptq2_solver.h -- geometric-multigrid preconditioned Krylov solve of the viscous
+ * block A11 u = f on one GPU (libptq2.so).
+ *
+ * This replaces the PETSc configuration that pTatin3D builds around its A11 shells:
+ * PCMG over a DMDA hierarchy coarsened by 2 (pTatin3dStokesKSPConfigureFSGMG,
+ * src/ptatin_driver_linear_ts.c:820-883), Chebyshev smoothing with Jacobi
+ * (KSPSolve_ChebychevRN_lag_norm, src/ksp_chebyrn.c:278-447; eigen-estimate transform
+ * of KSPChebychevSetEstimateEigenvalues, :178-214), trilinear DMDA interpolation on the
+ * Q2 node lattice with injected coordinates (src/dmda_project_coords.c:39-71), an
+ * assembled FP64 coarse level (src/ptatin_driver_linear_ts.c:630-633) and FGMRES outside.
+ *
+ * Precision: smoothing runs in FP32 (the FP32 A11 operator with FP32 vectors) or in
+ * FP64; the Krylov residual, the interlevel FP64 data and the coarse solve are FP64.
+ * The whole V-cycle is replayed from a captured hipGraph.
+ */
+#ifndef PTQ2_SOLVER_H
+#define PTQ2_SOLVER_H
+#include <stdint.h>
+
+#include "ptq2.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ptq2_mg ptq2_mg;
+
+typedef struct {
+  int nlevels;          /* levels including the fine one (>= 2); each coarsens by 2 */
+  int smooth_its;       /* Chebyshev iterations per pre/post smoothing */
+  int fp32_smoother;    /* 1: FP32 smoothing on every non-coarse level, 0: FP64 */
+  double emin_frac;     /* emin = emin_frac * emax_estimate (tform[1]) */
+  double emax_frac;     /* emax = emax_frac * emax_estimate (tform[3]) */
+  int esteig_its;       /* power iterations for the estimate of lambda_max(D^-1 A) */
+  int use_graph;        /* 1: replay the V-cycle from a captured hipGraph */
+  int coarse_max_dofs;  /* refuse a dense coarse level larger than this */
+} ptq2_mg_opts;
+
+/* fill opts with the defaults: Chebyshev(4), FP32 smoothing, (0.2, 1.1) transform,
+   20 power iterations, graph replay, coarse dense limit 6000 dofs */
+void ptq2_mg_default_opts(ptq2_mg_opts *opts);
+
+/* Build the hierarchy under a set-up context (mesh with BC mask + viscosity): coarse
+   meshes by coordinate injection, BC masks by injection, quadrature-point viscosity
+   by averaging over the 8 children of each coarse element, Jacobi diagonals,
+   eigen-estimates, the dense FP64 inverse of the coarsest operator and the graph. */
+int ptq2_mg_create(ptq2_mg **mg, ptq2_ctx *fine, const ptq2_mg_opts *opts);
+int ptq2_mg_destroy(ptq2_mg **mg);
+int ptq2_mg_levels(const ptq2_mg *mg, int *nlevels, int *m_per_level /* [nlevels] x,y,z triples */);
+
+/* the Chebyshev interval [emin, emax] of every smoothed level (nlevels-1 entries each) */
+int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax);
+
+/* order all subsequent solver work (every level and the captured V-cycle) on
+   `hip_stream` (NULL = the null stream) */
+int ptq2_mg_set_stream(ptq2_mg *mg, void *hip_stream);
+
+/* one V-cycle z = M^-1 r on device FP64 vectors (stream ordered) */
+int ptq2_mg_vcycle(ptq2_mg *mg, const double *r, double *z);
+
+/* FGMRES(restart) with the V-cycle as right preconditioner on device FP64 vectors:
+   solves MatMult_MFStokes_A11 x = b from x = 0 until ||b - A x|| <= rtol ||b||.
+   Returns the iteration count and the final true residual norm. */
+int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_it, int restart,
+                  int *iterations, double *final_rnorm, double *rnorm0);
+
+/* residual history of the last solve (one norm per iteration, up to cap entries) */
+int ptq2_mg_history(const ptq2_mg *mg, double *hist, int cap, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PTQ2_SOLVER_H */

@@ -1,0 +1,218 @@
+"""CPU restatement of the multigrid V-cycle used by libptq2's viscous solver.
+
+TEST INFRASTRUCTURE ONLY (the checker of tests/test_gpu_solver.py).  Follows:
+  - the ChebychevRN recurrence of KSPSolve_ChebychevRN_lag_norm (src/ksp_chebyrn.c:278-447)
+    with Jacobi (the A11 diagonal of MatGetDiagonal_MFStokes_A11);
+  - coordinate injection onto the coarse node lattice (src/dmda_project_coords.c:39-71);
+  - trilinear interpolation of the Q2 node lattice (PETSc DMDA Q1 interpolation,
+    src/ptatin_driver_linear_ts.c:996-1006) and restriction = its transpose, with the
+    corrections and restricted residuals zeroed on Dirichlet dofs;
+  - coarse quadrature-point viscosity = mean of the fine quadrature values of the child
+    elements a coarse point touches;
+  - an exact (dense) solve on the coarsest level.
+The element operators are the C oracle's (oracle/ptq2_oracle.c).
+"""
+import numpy as np
+
+from . import oracle as O
+
+
+def eta_coarsen(mx, my, mz, eta, mode="arithmetic"):
+    """coarse (mx/2,...) quadrature values from the fine ones (arithmetic or geometric mean)"""
+    if mode == "geometric":
+        return np.exp(eta_coarsen(mx, my, mz, np.log(eta), "arithmetic"))
+    e = np.asarray(eta).reshape(mz, my, mx, 3, 3, 3)  # (ez,ey,ex, qc,qb,qa)
+    mxc, myc, mzc = mx // 2, my // 2, mz // 2
+    out = np.zeros((mzc, myc, mxc, 3, 3, 3))
+    child_mean = e.reshape(mzc, 2, myc, 2, mxc, 2, 27).mean(axis=6)  # (cz,c, cy,b, cx,a)
+    for qc in range(3):
+        cs = [0] if qc == 0 else ([1] if qc == 2 else [0, 1])
+        for qb in range(3):
+            bs = [0] if qb == 0 else ([1] if qb == 2 else [0, 1])
+            for qa in range(3):
+                as_ = [0] if qa == 0 else ([1] if qa == 2 else [0, 1])
+                acc = 0.0
+                for c in cs:
+                    for b in bs:
+                        for a in as_:
+                            acc = acc + child_mean[:, c, :, b, :, a]
+                out[:, :, :, qc, qb, qa] = acc / (len(cs) * len(bs) * len(as_))
+    return out.reshape(-1)
+
+
+def prolong(mxc, myc, mzc, xc):
+    """trilinear interpolation of a coarse nodal 3-vector field to the fine lattice"""
+    c = np.asarray(xc).reshape(2 * mzc + 1, 2 * myc + 1, 2 * mxc + 1, 3)
+
+    def up(a, axis):
+        n = a.shape[axis]
+        shape = list(a.shape)
+        shape[axis] = 2 * n - 1
+        out = np.zeros(shape)
+        sl = [slice(None)] * a.ndim
+        sl[axis] = slice(0, None, 2)
+        out[tuple(sl)] = a
+        lo = [slice(None)] * a.ndim
+        hi = [slice(None)] * a.ndim
+        mid = [slice(None)] * a.ndim
+        lo[axis] = slice(0, n - 1)
+        hi[axis] = slice(1, n)
+        mid[axis] = slice(1, None, 2)
+        out[tuple(mid)] = 0.5 * (a[tuple(lo)] + a[tuple(hi)])
+        return out
+
+    f = up(up(up(c, 2), 1), 0)
+    return f.reshape(-1)
+
+
+def restrict(mxc, myc, mzc, xf):
+    """transpose of prolong"""
+    nc = 3 * (2 * mxc + 1) * (2 * myc + 1) * (2 * mzc + 1)
+    # explicit transpose through the linear map on unit vectors would be slow; use the adjoint stencil
+    f = np.asarray(xf).reshape(4 * mzc + 1, 4 * myc + 1, 4 * mxc + 1, 3)
+
+    def down(a, axis):
+        n = a.shape[axis]
+        nc_ = (n + 1) // 2
+        shape = list(a.shape)
+        shape[axis] = nc_
+        out = np.zeros(shape)
+        ev = [slice(None)] * a.ndim
+        ev[axis] = slice(0, None, 2)
+        out += a[tuple(ev)]
+        od = [slice(None)] * a.ndim
+        od[axis] = slice(1, None, 2)
+        odd = 0.5 * a[tuple(od)]
+        lo = [slice(None)] * a.ndim
+        hi = [slice(None)] * a.ndim
+        lo[axis] = slice(0, nc_ - 1)
+        hi[axis] = slice(1, nc_)
+        out[tuple(lo)] += odd
+        out[tuple(hi)] += odd
+        return out
+
+    c = down(down(down(f, 0), 1), 2)
+    assert c.size == nc
+    return c.reshape(-1)
+
+
+# 1D quadratic prolongation of the Q2 node lattice: fine node 4E+t of coarse element E sits at
+# xi = -1 + t/2 and takes the coarse Q2 basis values there.
+_Q2W = {0: ((0, 1.0),), 1: ((0, 0.375), (1, 0.75), (2, -0.125)), 2: ((1, 1.0),),
+        3: ((0, -0.125), (1, 0.75), (2, 0.375))}
+
+
+def _p1d_q2(nc):
+    """dense 1D quadratic prolongation, (2*nc-1) x nc, nc = 2*mc+1 coarse lattice nodes"""
+    nf = 2 * nc - 1
+    P = np.zeros((nf, nc))
+    for f in range(nf):
+        if f == nf - 1:
+            P[f, nc - 1] = 1.0
+            continue
+        E, t = divmod(f, 4)
+        for off, w in _Q2W[t]:
+            P[f, 2 * E + off] += w
+    return P
+
+
+def prolong_q2(mxc, myc, mzc, xc):
+    c = np.asarray(xc).reshape(2 * mzc + 1, 2 * myc + 1, 2 * mxc + 1, 3)
+    Px, Py, Pz = _p1d_q2(2 * mxc + 1), _p1d_q2(2 * myc + 1), _p1d_q2(2 * mzc + 1)
+    f = np.einsum("ai,bj,ck,kjid->cbad", Px, Py, Pz, c, optimize=True)
+    return f.reshape(-1)
+
+
+def restrict_q2(mxc, myc, mzc, xf):
+    f = np.asarray(xf).reshape(4 * mzc + 1, 4 * myc + 1, 4 * mxc + 1, 3)
+    Px, Py, Pz = _p1d_q2(2 * mxc + 1), _p1d_q2(2 * myc + 1), _p1d_q2(2 * mzc + 1)
+    c = np.einsum("ai,bj,ck,cbad->kjid", Px, Py, Pz, f, optimize=True)
+    return c.reshape(-1)
+
+
+class Level:
+    def __init__(self, mx, my, mz, coords, mask, eta):
+        self.mx, self.my, self.mz = mx, my, mz
+        self.coords, self.mask, self.eta = coords, mask, eta
+        self.n = coords.size
+
+    def matmult(self, x):
+        return O.matmult_a11(self.mx, self.my, self.mz, self.coords, self.eta, self.mask, x)
+
+    def diag(self):
+        return O.matgetdiag_a11(self.mx, self.my, self.mz, self.coords, self.eta, self.mask)
+
+
+def hierarchy(mx, my, mz, coords, mask, eta, nlevels, mode="arithmetic"):
+    lv = [Level(mx, my, mz, coords, mask, eta)]
+    for _ in range(nlevels - 1):
+        f = lv[-1]
+        mask = None if f.mask is None else inject_from_fine(f, f.mask)
+        c = Level(f.mx // 2, f.my // 2, f.mz // 2, inject_from_fine(f), mask, eta_coarsen(f.mx, f.my, f.mz, f.eta, mode))
+        lv.append(c)
+    return lv
+
+
+def inject_from_fine(f, vals=None):
+    v = f.coords if vals is None else vals
+    g = np.asarray(v).reshape(2 * f.mz + 1, 2 * f.my + 1, 2 * f.mx + 1, 3)
+    return np.ascontiguousarray(g[::2, ::2, ::2]).reshape(-1)
+
+
+def chebyshev(L, b, x, dinv, emin, emax, its, zero_guess):
+    """KSPSolve_ChebychevRN_lag_norm with KSP_NORM_NONE (src/ksp_chebyrn.c:278-447)."""
+    scale = 2.0 / (emax + emin)
+    alpha = 1.0 - scale * emin
+    mu = 1.0 / alpha
+    omegaprod = 2.0 / alpha
+    p = [x if not zero_guess else np.zeros_like(b), None, None]
+    c = [1.0, mu, 0.0]
+    km1, k, kp1 = 0, 1, 2
+    r = b.copy() if zero_guess else b - L.matmult(x)
+    p[k] = scale * dinv * r + p[km1]
+    for _ in range(its):
+        c[kp1] = 2.0 * mu * c[k] - c[km1]
+        omega = omegaprod * c[k] / c[kp1]
+        r = b - L.matmult(p[k])
+        p[kp1] = omega * scale * dinv * r + (1.0 - omega) * p[km1] + omega * p[k]
+        km1, k, kp1 = k, kp1, km1
+    return p[k]
+
+
+def dense_inverse(L):
+    n = L.n
+    A = np.zeros((n, n))
+    e = np.zeros(n)
+    for j in range(n):
+        e[:] = 0.0
+        e[j] = 1.0
+        A[:, j] = L.matmult(e)
+    A = 0.5 * (A + A.T)
+    return np.linalg.inv(A)
+
+
+class VCycle:
+    def __init__(self, levels, emin, emax, its, transfer="trilinear"):
+        self.lv = levels
+        self.P = prolong_q2 if transfer == "q2" else prolong
+        self.R = restrict_q2 if transfer == "q2" else restrict
+        self.emin, self.emax, self.its = emin, emax, its
+        self.dinv = [1.0 / L.diag() for L in levels[:-1]]
+        self.ainv = dense_inverse(levels[-1])
+
+    def __call__(self, b, l=0):
+        L = self.lv[l]
+        if l == len(self.lv) - 1:
+            return self.ainv @ b
+        x = chebyshev(L, b, None, self.dinv[l], self.emin[l], self.emax[l], self.its, True)
+        r = b - L.matmult(x)
+        C = self.lv[l + 1]
+        bc = self.R(C.mx, C.my, C.mz, r)
+        if C.mask is not None:
+            bc[C.mask > 0] = 0.0
+        xc = self(bc, l + 1)
+        corr = self.P(C.mx, C.my, C.mz, xc)
+        if L.mask is not None:
+            corr[L.mask > 0] = 0.0
+        x = x + corr
+        return chebyshev(L, b, x, self.dinv[l], self.emin[l], self.emax[l], self.its, False)

@@ -19,33 +19,9 @@
 #include "ptq2.h"
 #include "ptq2_device.h"
 #include "ptq2_kernels.h"
+#include "ptq2_ctx.h"
 
 using ptq2::Box;
-
-struct ptq2_ctx {
-  int device = 0;
-  hipStream_t own_stream = nullptr;
-  hipStream_t stream = nullptr;
-  bool mesh_ready = false, eta_ready = false;
-  Box box{};
-  int64_t nnodes = 0, nvel = 0, nel = 0, npres = 0, nbricks = 0;
-  double *coords = nullptr;
-  uint8_t *mask = nullptr;
-  int64_t *dir_idx = nullptr;
-  int64_t ndir = 0;
-  double *eta_bm = nullptr;
-  float *fac32 = nullptr;
-  bool fac32_valid = false;
-  bool has_state = false;
-  uint64_t state = 0;
-  int *err = nullptr;
-  // staging for host-pointer calls
-  void *su = nullptr, *sy = nullptr;
-  size_t su_bytes = 0, sy_bytes = 0;
-  double *sp = nullptr, *syp = nullptr;
-  size_t sp_bytes = 0, syp_bytes = 0;
-  std::string last;
-};
 
 namespace {
 
@@ -110,7 +86,10 @@ int clear_err(ptq2_ctx *c) {
   return PTQ2_OK;
 }
 
-int build_fac32(ptq2_ctx *c) {
+}  // namespace
+
+int ptq2_internal_build_fac32(ptq2_ctx *c);
+int ptq2_internal_build_fac32(ptq2_ctx *c) {
   if (c->fac32_valid) return PTQ2_OK;
   if (!c->fac32) {
     int rc = dev_alloc(c, (void **)&c->fac32, sizeof(float) * (size_t)c->nbricks * 243 * ptq2::BEL);
@@ -127,6 +106,10 @@ int build_fac32(ptq2_ctx *c) {
   c->fac32_valid = true;
   return PTQ2_OK;
 }
+
+namespace {
+
+int build_fac32(ptq2_ctx *c) { return ptq2_internal_build_fac32(c); }
 
 int check_ready(ptq2_ctx *c, bool need_eta) {
   if (!c) return PTQ2_ERR_INVALID_ARG;
@@ -211,9 +194,11 @@ int ptq2_destroy(ptq2_ctx **pc) {
 
 int ptq2_set_stream(ptq2_ctx *c, void *s) {
   if (!c) return PTQ2_ERR_INVALID_ARG;
-  c->stream = s ? (hipStream_t)s : c->own_stream;
+  c->stream = (hipStream_t)s;
   return PTQ2_OK;
 }
+
+void *ptq2_get_stream(const ptq2_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
 int ptq2_sync(ptq2_ctx *c) {
   if (!c) return PTQ2_ERR_INVALID_ARG;

@@ -1,0 +1,37 @@
+// ptq2_ctx.h -- the context behind the opaque ptq2_ctx handle (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "ptq2_device.h"
+
+struct ptq2_ctx {
+  int device = 0;
+  hipStream_t own_stream = nullptr;
+  hipStream_t stream = nullptr;
+  bool mesh_ready = false, eta_ready = false;
+  ptq2::Box box{};
+  int64_t nnodes = 0, nvel = 0, nel = 0, npres = 0, nbricks = 0;
+  double *coords = nullptr;
+  uint8_t *mask = nullptr;
+  int64_t *dir_idx = nullptr;
+  int64_t ndir = 0;
+  double *eta_bm = nullptr;
+  float *fac32 = nullptr;
+  bool fac32_valid = false;
+  bool has_state = false;
+  uint64_t state = 0;
+  int *err = nullptr;
+  // staging for host-pointer calls
+  void *su = nullptr, *sy = nullptr;
+  size_t su_bytes = 0, sy_bytes = 0;
+  double *sp = nullptr, *syp = nullptr;
+  size_t sp_bytes = 0, syp_bytes = 0;
+  std::string last;
+};
+
+
+// (re)build the FP32 geometric factors of the smoother operator if stale
+int ptq2_internal_build_fac32(ptq2_ctx *c);

@@ -327,8 +327,106 @@ __device__ __forceinline__ void acc_direct(const T dv[3][3], T RB[3][3][3], T RD
       }
 }
 
-// keep the scheduler from hoisting the next row's LDS reads into this one
+// ---------------------------------------------------------------------------
+// Runtime-row variants.  The quadrature row (qb,qc) is a runtime (wave-uniform) loop
+// index; its 1D coefficients come from constant memory into scalar registers, so the
+// whole element body is one row-sized block of code (a fully unrolled element does
+// not fit the instruction cache).  Inside a row everything is unrolled.
 __device__ __forceinline__ void row_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+struct Coef1 {
+  double b[3], d[3];
+};
+__constant__ const double c_gB[3][3] = {
+    {0.5 * (0.6 + 0.77459666924148337704), 1.0 - 0.6, 0.5 * (0.6 - 0.77459666924148337704)},
+    {0.0, 1.0, 0.0},
+    {0.5 * (0.6 - 0.77459666924148337704), 1.0 - 0.6, 0.5 * (0.6 + 0.77459666924148337704)}};
+__constant__ const double c_gD[3][3] = {{-0.77459666924148337704 - 0.5, 2.0 * 0.77459666924148337704, -0.77459666924148337704 + 0.5},
+                                         {-0.5, 0.0, 0.5},
+                                         {0.77459666924148337704 - 0.5, -2.0 * 0.77459666924148337704, 0.77459666924148337704 + 0.5}};
+__constant__ const double c_gw[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+
+__device__ __forceinline__ Coef1 coef1(int q) {
+  Coef1 r;
+#pragma unroll
+  for (int n = 0; n < 3; n++) {
+    r.b[n] = c_gB[q][n];
+    r.d[n] = c_gD[q][n];
+  }
+  return r;
+}
+
+// row data for runtime (qb,qc): z contraction first, then y.
+template <typename T>
+__device__ __forceinline__ void row_data_rt(const T *s, int base, const Coef1 &cb, const Coef1 &cc, T r0[3][3],
+                                            T r1[3][3], T r2[3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      T tb[3], td[3];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        T v0 = tv(s, base, c, i, j, 0), v1 = tv(s, base, c, i, j, 1), v2 = tv(s, base, c, i, j, 2);
+        tb[j] = fmac(cc.b[2], v2, fmac(cc.b[1], v1, static_cast<T>(cc.b[0]) * v0));
+        td[j] = fmac(cc.d[2], v2, fmac(cc.d[1], v1, static_cast<T>(cc.d[0]) * v0));
+      }
+      r0[c][i] = fmac(cb.b[2], tb[2], fmac(cb.b[1], tb[1], static_cast<T>(cb.b[0]) * tb[0]));
+      r1[c][i] = fmac(cb.d[2], tb[2], fmac(cb.d[1], tb[1], static_cast<T>(cb.d[0]) * tb[0]));
+      r2[c][i] = fmac(cb.b[2], td[2], fmac(cb.b[1], td[1], static_cast<T>(cb.b[0]) * td[0]));
+      __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// y-transpose of a finished row (runtime qb) into the slab accumulators
+template <typename T>
+__device__ __forceinline__ void acc_slab_rt(const Coef1 &cb, const T P0[3][3], const T P1[3][3], const T P2[3][3],
+                                            T RB[3][3][3], T RD[3][3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        RB[c][j][i] = fmac(cb.b[j], P0[c][i], fmac(cb.d[j], P1[c][i], RB[c][j][i]));
+        RD[c][j][i] = fmac(cb.b[j], P2[c][i], RD[c][j][i]);
+      }
+}
+
+// z-transpose of a finished slab (runtime qc) into the element output
+template <typename T>
+__device__ __forceinline__ void acc_elem_rt(const Coef1 &cc, const T RB[3][3][3], const T RD[3][3][3],
+                                            T Y[3][3][3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+          Y[c][k][j][i] = fmac(cc.b[k], RB[c][j][i], fmac(cc.d[k], RD[c][j][i], Y[c][k][j][i]));
+}
+
+// a finished row (runtime qb,qc) straight into the element output: per (c,j,i) the
+// y-transposed pair is formed once and spread over the three z nodes.
+template <typename T>
+__device__ __forceinline__ void acc_row_elem_rt(const Coef1 &cb, const Coef1 &cc, const T P0[3][3],
+                                                const T P1[3][3], const T P2[3][3], T Y[3][3][3][3]) {
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const T tb = fmac(cb.b[j], P0[c][i], static_cast<T>(cb.d[j]) * P1[c][i]);
+        const T td = static_cast<T>(cb.b[j]) * P2[c][i];
+#pragma unroll
+        for (int k = 0; k < 3; k++) Y[c][k][j][i] = fmac(cc.b[k], tb, fmac(cc.d[k], td, Y[c][k][j][i]));
+      }
+}
+
+// keep the scheduler from hoisting the next row's LDS reads into this one
 
 // z-transpose of a finished slab into the element output Y[c][k][j][i]
 template <int qc, typename T>

@@ -11,52 +11,40 @@ namespace ptq2 {
 
 // ----------------------------------------------------------------- A11 FP64
 
-// geometry of one quadrature point of row (qb,qc): adjugate and scale
-// s = w eta / det, from the coordinate row data.  eta9 holds the slab's 9 values.
-template <int qa, int qb, int qc>
+// geometry of quadrature point qa of the current row: adjugate and s = w eta / det
+template <int qa>
 __device__ __forceinline__ void geo_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
-                                       const double *eta9, double A[3][3], double &s) {
+                                       double wbc, double eta, double A[3][3], double &s) {
   double J[3][3];
   qp_grad<qa>(xr0, xr1, xr2, J);
   double det = adjugate(J, A);
-  constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
-  s = (w * eta9[qa + 3 * qb]) * rcp64(det);
+  s = (gw1(qa) * wbc * eta) * rcp64(det);
 }
 
-template <int qb, int qc>
-__device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, int base, const double *eta9,
-                                          double RB[3][3][3], double RD[3][3][3]) {
+// one quadrature row (runtime qb,qc) of the FP64 A11 element: geometry of its three
+// points from the coordinate tile, velocity gradients from the velocity tile, the
+// stress action and the x-transpose into P, then the y-transpose into the slab sums.
+__device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, int base, const Coef1 &cb,
+                                          const Coef1 &cc, double wbc, const double e3[3], double Y[3][3][3][3]) {
   double A0[3][3], A1[3][3], A2[3][3], s0, s1, s2;
   {
     double xr0[3][3], xr1[3][3], xr2[3][3];
-    row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
-    geo_qp<0, qb, qc>(xr0, xr1, xr2, eta9, A0, s0);
-    geo_qp<1, qb, qc>(xr0, xr1, xr2, eta9, A1, s1);
-    geo_qp<2, qb, qc>(xr0, xr1, xr2, eta9, A2, s2);
+    row_data_rt(Xs, base, cb, cc, xr0, xr1, xr2);
+    geo_qp<0>(xr0, xr1, xr2, wbc, e3[0], A0, s0);
+    geo_qp<1>(xr0, xr1, xr2, wbc, e3[1], A1, s1);
+    geo_qp<2>(xr0, xr1, xr2, wbc, e3[2], A2, s2);
   }
-  double ur0[3][3], ur1[3][3], ur2[3][3];
-  row_data<qb, qc>(Us, base, ur0, ur1, ur2);
-  double g[3][3], dv[3][3];
-  qp_grad<0>(ur0, ur1, ur2, g); stress_action(A0, g, s0, dv); acc_direct<0, qb>(dv, RB, RD);
-  qp_grad<1>(ur0, ur1, ur2, g); stress_action(A1, g, s1, dv); acc_direct<1, qb>(dv, RB, RD);
-  qp_grad<2>(ur0, ur1, ur2, g); stress_action(A2, g, s2, dv); acc_direct<2, qb>(dv, RB, RD);
   row_fence();
-}
-
-template <int qc>
-__device__ __forceinline__ void a11_slab64(const double *Xs, const double *Us, int base, const double *eta9,
-                                           double Y[3][3][3][3]) {
-  double RB[3][3][3], RD[3][3][3];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0;
-  a11_row64<0, qc>(Xs, Us, base, eta9, RB, RD);
-  a11_row64<1, qc>(Xs, Us, base, eta9, RB, RD);
-  a11_row64<2, qc>(Xs, Us, base, eta9, RB, RD);
-  acc_elem<qc>(RB, RD, Y);
+  double ur0[3][3], ur1[3][3], ur2[3][3];
+  row_data_rt(Us, base, cb, cc, ur0, ur1, ur2);
+  double P0[3][3], P1[3][3], P2[3][3];
+  zero33(P0); zero33(P1); zero33(P2);
+  double g[3][3], dv[3][3];
+  qp_grad<0>(ur0, ur1, ur2, g); stress_action(A0, g, s0, dv); acc_row<0>(dv, P0, P1, P2);
+  qp_grad<1>(ur0, ur1, ur2, g); stress_action(A1, g, s1, dv); acc_row<1>(dv, P0, P1, P2);
+  qp_grad<2>(ur0, ur1, ur2, g); stress_action(A2, g, s2, dv); acc_row<2>(dv, P0, P1, P2);
+  row_fence();
+  acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
   row_fence();
 }
 
@@ -77,6 +65,11 @@ __device__ __forceinline__ void load9(const double *__restrict__ ep, int qc, dou
   for (int q = 0; q < 9; q++) e9[q] = ep[(9 * qc + q) * BEL];
 }
 
+__device__ __forceinline__ void load3(const double *__restrict__ ep, int row, double *e3) {
+#pragma unroll
+  for (int q = 0; q < 3; q++) e3[q] = ep[(3 * row + q) * BEL];
+}
+
 __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double *__restrict__ coords,
                                                   const double *__restrict__ u, const uint8_t *__restrict__ mask,
                                                   const double *__restrict__ eta_bm, double *__restrict__ y,
@@ -87,8 +80,8 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
   const BrickId br = brick_of(b, color, blockIdx.x);
   const Tile t = tile_of(b, br);
   const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
-  double ea[9], eb[9];
-  load9(ep, 0, ea);
+  double en[3];
+  load3(ep, 0, en);
   load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
   load_tile(b, t, u, mask, Us, lane);
   __syncthreads();
@@ -104,11 +97,13 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
   if (active) {
-    load9(ep, 1, eb);
-    a11_slab64<0>(Xs, Us, base, ea, Y);
-    load9(ep, 2, ea);
-    a11_slab64<1>(Xs, Us, base, eb, Y);
-    a11_slab64<2>(Xs, Us, base, ea, Y);
+#pragma nounroll
+    for (int row = 0; row < 9; row++) {
+      const int qb = row % 3, qc = row / 3;
+      double e3[3] = {en[0], en[1], en[2]};
+      if (row < 8) load3(ep, row + 1, en);
+      a11_row64(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, Y);
+    }
   }
   __syncthreads();
   zero_tile(Xs, lane);
@@ -124,13 +119,13 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
 // [brick][q][m][lane] so that every load instruction is one contiguous 256-byte
 // segment.  Then dv = F^T (M + M^T) with M = F g (diagonal doubled), all FP32.
 
-template <int qa, int qb, int qc>
+template <int qa>
 __device__ __forceinline__ void a11_qp32(const float ur0[3][3], const float ur1[3][3], const float ur2[3][3],
-                                         const float *__restrict__ fp, float RB[3][3][3], float RD[3][3][3]) {
-  constexpr int q = qa + 3 * qb + 9 * qc;
+                                         const float *__restrict__ fp, float P0[3][3], float P1[3][3],
+                                         float P2[3][3]) {
   float F[3][3];
 #pragma unroll
-  for (int m = 0; m < 9; m++) F[m / 3][m % 3] = fp[(q * 9 + m) * BEL];
+  for (int m = 0; m < 9; m++) F[m / 3][m % 3] = fp[(qa * 9 + m) * BEL];
   float g[3][3], dv[3][3];
   qp_grad<qa>(ur0, ur1, ur2, g);
   float M[3][3];
@@ -149,35 +144,7 @@ __device__ __forceinline__ void a11_qp32(const float ur0[3][3], const float ur1[
   for (int k = 0; k < 3; k++)
 #pragma unroll
     for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * F[0][k] + S[1][l] * F[1][k] + S[2][l] * F[2][k];
-  acc_direct<qa, qb>(dv, RB, RD);
-}
-
-template <int qb, int qc>
-__device__ __forceinline__ void a11_row32(const float *Us, int base, const float *__restrict__ fp, float RB[3][3][3],
-                                          float RD[3][3][3]) {
-  float ur0[3][3], ur1[3][3], ur2[3][3];
-  row_data<qb, qc>(Us, base, ur0, ur1, ur2);
-  a11_qp32<0, qb, qc>(ur0, ur1, ur2, fp, RB, RD);
-  a11_qp32<1, qb, qc>(ur0, ur1, ur2, fp, RB, RD);
-  a11_qp32<2, qb, qc>(ur0, ur1, ur2, fp, RB, RD);
-  row_fence();
-}
-
-template <int qc>
-__device__ __forceinline__ void a11_slab32(const float *Us, int base, const float *__restrict__ fp,
-                                           float Y[3][3][3][3]) {
-  float RB[3][3][3], RD[3][3][3];
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0f;
-  a11_row32<0, qc>(Us, base, fp, RB, RD);
-  a11_row32<1, qc>(Us, base, fp, RB, RD);
-  a11_row32<2, qc>(Us, base, fp, RB, RD);
-  acc_elem<qc>(RB, RD, Y);
-  row_fence();
+  acc_row<qa>(dv, P0, P1, P2);
 }
 
 __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
@@ -191,7 +158,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   __syncthreads();
   const int base = lane_base(lane);
   const bool active = lane_active(t, lane);
-  const float *fp = fac + (int64_t)br.lin * 243 * BEL + lane;
+  const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
   float Y[3][3][3][3];
 #pragma unroll
   for (int c = 0; c < 3; c++)
@@ -202,9 +169,31 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
   if (active) {
-    a11_slab32<0>(Us, base, fp, Y);
-    a11_slab32<1>(Us, base, fp, Y);
-    a11_slab32<2>(Us, base, fp, Y);
+#pragma nounroll
+    for (int qc = 0; qc < 3; qc++) {
+      const Coef1 cc = coef1(qc);
+      float RB[3][3][3], RD[3][3][3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0f;
+#pragma nounroll
+      for (int qb = 0; qb < 3; qb++) {
+        const Coef1 cb = coef1(qb);
+        const float *fp = fbase + (int64_t)(3 * (qb + 3 * qc)) * 9 * BEL;
+        float ur0[3][3], ur1[3][3], ur2[3][3];
+        row_data_rt(Us, base, cb, cc, ur0, ur1, ur2);
+        float P0[3][3], P1[3][3], P2[3][3];
+        zero33(P0); zero33(P1); zero33(P2);
+        a11_qp32<0>(ur0, ur1, ur2, fp, P0, P1, P2);
+        a11_qp32<1>(ur0, ur1, ur2, fp, P0, P1, P2);
+        a11_qp32<2>(ur0, ur1, ur2, fp, P0, P1, P2);
+        acc_slab_rt(cb, P0, P1, P2, RB, RD);
+      }
+      acc_elem_rt(cc, RB, RD, Y);
+    }
   }
   __syncthreads();
   zero_tile(Us, lane);

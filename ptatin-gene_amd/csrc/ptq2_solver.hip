@@ -1,0 +1,893 @@
+// ptq2_solver.hip -- GMG-preconditioned FGMRES for the viscous block (include/ptq2_solver.h).
+//
+// Level l has its own ptq2_ctx (mesh, BC mask, viscosity).  Coarse meshes are the
+// fine node lattice injected every second node (src/dmda_project_coords.c:39-71);
+// transfers are the trilinear interpolation of the Q2 node lattice (DMDA Q1
+// interpolation, src/ptatin_driver_linear_ts.c:996-1006) and its transpose;
+// smoothing is the ChebychevRN recurrence (src/ksp_chebyrn.c:278-447) with the
+// A11 diagonal as Jacobi preconditioner; the coarsest level is solved with a dense
+// FP64 inverse.  All level work of one V-cycle is captured once into a hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "ptq2.h"
+#include "ptq2_ctx.h"
+#include "ptq2_kernels.h"
+#include "ptq2_solver.h"
+
+using ptq2::Box;
+
+namespace {
+
+constexpr int TPB = 256;
+
+inline unsigned nblk(int64_t n, int64_t cap = 16384) {
+  int64_t b = (n + TPB - 1) / TPB;
+  return (unsigned)std::max<int64_t>(1, std::min(b, cap));
+}
+
+// ------------------------------------------------------------ vector kernels
+template <typename T>
+__global__ void k_fill(int64_t n, T *x, T v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
+}
+
+template <typename S, typename D>
+__global__ void k_convert(int64_t n, const S *__restrict__ a, D *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = static_cast<D>(a[i]);
+}
+
+// p1 = scale * Dinv * r0 + x, with r0 = b (zero guess) or b - y
+template <typename T>
+__global__ void k_cheb_first(int64_t n, const T *__restrict__ b, const T *__restrict__ y, const T *__restrict__ dinv,
+                             const T *__restrict__ x, T *__restrict__ p, T scale) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T r = y ? b[i] - y[i] : b[i];
+    T xi = x ? x[i] : T(0);
+    p[i] = scale * dinv[i] * r + xi;
+  }
+}
+
+// fused Chebyshev step: p_{k+1} = omega*scale*Dinv*(b - A p_k) + (1-omega) p_{k-1} + omega p_k
+template <typename T>
+__global__ void k_cheb_step(int64_t n, const T *__restrict__ b, const T *__restrict__ y, const T *__restrict__ dinv,
+                            const T *__restrict__ pkm1, const T *__restrict__ pk, T *__restrict__ pkp1, T os, T om1,
+                            T om) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    pkp1[i] = os * dinv[i] * (b[i] - y[i]) + om1 * pkm1[i] + om * pk[i];
+}
+
+// restriction R = P^T of the residual b - y onto the coarse node lattice, zero on
+// coarse Dirichlet dofs.  One thread per coarse dof gathers its 3x3x3 fine stencil.
+template <typename T>
+__global__ void k_restrict(int NXc, int NYc, int NZc, int NXf, int NYf, int NZf, const T *__restrict__ b,
+                           const T *__restrict__ y, const uint8_t *__restrict__ cmask, T *__restrict__ bc) {
+  int64_t nc = 3 * (int64_t)NXc * NYc * NZc;
+  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < nc; d += (int64_t)gridDim.x * blockDim.x) {
+    int comp = (int)(d % 3);
+    int64_t node = d / 3;
+    int I = (int)(node % NXc), J = (int)((node / NXc) % NYc), K = (int)(node / ((int64_t)NXc * NYc));
+    T acc = T(0);
+    if (!(cmask && cmask[d])) {
+      for (int dk = -1; dk <= 1; dk++) {
+        int k = 2 * K + dk;
+        if (k < 0 || k >= NZf) continue;
+        T wk = dk == 0 ? T(1) : T(0.5);
+        for (int dj = -1; dj <= 1; dj++) {
+          int j = 2 * J + dj;
+          if (j < 0 || j >= NYf) continue;
+          T wj = dj == 0 ? T(1) : T(0.5);
+          for (int di = -1; di <= 1; di++) {
+            int i = 2 * I + di;
+            if (i < 0 || i >= NXf) continue;
+            T wi = di == 0 ? T(1) : T(0.5);
+            int64_t f = 3 * ((int64_t)i + (int64_t)NXf * ((int64_t)j + (int64_t)NYf * k)) + comp;
+            T r = y ? b[f] - y[f] : b[f];
+            acc += wk * wj * wi * r;
+          }
+        }
+      }
+    }
+    bc[d] = acc;
+  }
+}
+
+// x_f += P x_c (trilinear), not on fine Dirichlet dofs
+template <typename T>
+__global__ void k_prolong_add(int NXc, int NYc, int NXf, int NYf, int NZf, const T *__restrict__ xc,
+                              const uint8_t *__restrict__ fmask, T *__restrict__ xf) {
+  int64_t nf = 3 * (int64_t)NXf * NYf * NZf;
+  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < nf; d += (int64_t)gridDim.x * blockDim.x) {
+    if (fmask && fmask[d]) continue;
+    int comp = (int)(d % 3);
+    int64_t node = d / 3;
+    int i = (int)(node % NXf), j = (int)((node / NXf) % NYf), k = (int)(node / ((int64_t)NXf * NYf));
+    int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+    int i1 = (i & 1) ? i0 + 1 : i0, j1 = (j & 1) ? j0 + 1 : j0, k1 = (k & 1) ? k0 + 1 : k0;
+    T wi = (i & 1) ? T(0.5) : T(1), wj = (j & 1) ? T(0.5) : T(1), wk = (k & 1) ? T(0.5) : T(1);
+    auto at = [&](int a, int bb, int c) { return xc[3 * ((int64_t)a + (int64_t)NXc * ((int64_t)bb + (int64_t)NYc * c)) + comp]; };
+    T v = at(i0, j0, k0);
+    if (i & 1) v += at(i1, j0, k0);
+    T s = v * wi;
+    if (j & 1) {
+      T v2 = at(i0, j1, k0);
+      if (i & 1) v2 += at(i1, j1, k0);
+      s = (s + v2 * wi) * wj;
+    }
+    if (k & 1) {
+      T v3 = at(i0, j0, k1);
+      if (i & 1) v3 += at(i1, j0, k1);
+      T s3 = v3 * wi;
+      if (j & 1) {
+        T v4 = at(i0, j1, k1);
+        if (i & 1) v4 += at(i1, j1, k1);
+        s3 = (s3 + v4 * wi) * wj;
+      }
+      s = (s + s3) * wk;
+    }
+    xf[d] += s;
+  }
+}
+
+// dense coarse solve x = Ainv b (FP64 inverse, one wave per row)
+template <typename T>
+__global__ void k_dense_gemv(int n, const double *__restrict__ Ainv, const T *__restrict__ b, T *__restrict__ x) {
+  int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+  int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  const double *a = Ainv + (int64_t)row * n;
+  double s = 0.0;
+  for (int j = lane; j < n; j += 64) s += a[j] * (double)b[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) x[row] = static_cast<T>(s);
+}
+
+// coarse quadrature-point viscosity: the mean of the fine quadrature values of the
+// children that each coarse quadrature point touches (1, 2, 4 or 8 children).
+__global__ void k_eta_coarsen(Box f, int mxc, int myc, int mzc, const double *__restrict__ eta_bm_f,
+                              double *__restrict__ eta_c) {
+  int64_t nq = 27 * (int64_t)mxc * myc * mzc;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += (int64_t)gridDim.x * blockDim.x) {
+    int q = (int)(t % 27);
+    int64_t e = t / 27;
+    int cx = (int)(e % mxc), cy = (int)((e / mxc) % myc), cz = (int)(e / ((int64_t)mxc * myc));
+    int qa = q % 3, qb = (q / 3) % 3, qc = q / 9;
+    double s = 0.0;
+    int cnt = 0;
+    for (int c = 0; c < 2; c++) {
+      if ((qc == 0 && c == 1) || (qc == 2 && c == 0)) continue;
+      for (int bb = 0; bb < 2; bb++) {
+        if ((qb == 0 && bb == 1) || (qb == 2 && bb == 0)) continue;
+        for (int a = 0; a < 2; a++) {
+          if ((qa == 0 && a == 1) || (qa == 2 && a == 0)) continue;
+          int ex = 2 * cx + a, ey = 2 * cy + bb, ez = 2 * cz + c;
+          int64_t lin = (ex >> 2) + (int64_t)f.nbx * ((ey >> 2) + (int64_t)f.nby * (ez >> 2));
+          int lane = (ex & 3) + 4 * (ey & 3) + 16 * (ez & 3);
+          const double *src = eta_bm_f + lin * 27 * ptq2::BEL + lane;
+          for (int qq = 0; qq < 27; qq++) s += src[qq * ptq2::BEL];
+          cnt += 27;
+        }
+      }
+    }
+    eta_c[t] = s / cnt;
+  }
+}
+
+// partial dot products of w with nv vectors: part[v * nb + block]
+__global__ void k_mdot(int64_t n, const double *const *__restrict__ V, int nv, const double *__restrict__ w,
+                       double *__restrict__ part) {
+  __shared__ double red[TPB / 64];
+  for (int v = 0; v < nv; v++) {
+    const double *a = V[v];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+      s += a[i] * w[i];
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int k = 0; k < TPB / 64; k++) t += red[k];
+      part[(int64_t)v * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// w -= sum_v h[v] V[v]
+__global__ void k_maxpy(int64_t n, const double *const *__restrict__ V, int nv, const double *__restrict__ h,
+                        double *__restrict__ w) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = w[i];
+    for (int v = 0; v < nv; v++) s -= h[v] * V[v][i];
+    w[i] = s;
+  }
+}
+
+// x += sum_v h[v] V[v]
+__global__ void k_mcombine(int64_t n, const double *const *__restrict__ V, int nv, const double *__restrict__ h,
+                           double *__restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double s = x[i];
+    for (int v = 0; v < nv; v++) s += h[v] * V[v][i];
+    x[i] = s;
+  }
+}
+
+__global__ void k_scale_copy(int64_t n, const double *__restrict__ a, double s, double *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = s * a[i];
+}
+
+__global__ void k_sub(int64_t n, const double *__restrict__ b, const double *__restrict__ y, double *__restrict__ r) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    r[i] = b[i] - y[i];
+}
+
+// partial sums of v.w and v.(d v) per block: part[2*block], part[2*block+1]
+__global__ void k_rq_parts(int64_t n, const double *__restrict__ v, const double *__restrict__ w,
+                           const double *__restrict__ d, double *__restrict__ part) {
+  __shared__ double r0[TPB / 64], r1[TPB / 64];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    a += v[i] * w[i];
+    b += v[i] * d[i] * v[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    r0[threadIdx.x / 64] = a;
+    r1[threadIdx.x / 64] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t0 = 0.0, t1 = 0.0;
+    for (int k = 0; k < TPB / 64; k++) {
+      t0 += r0[k];
+      t1 += r1[k];
+    }
+    part[2 * blockIdx.x] = t0;
+    part[2 * blockIdx.x + 1] = t1;
+  }
+}
+
+__global__ void k_div(int64_t n, const double *__restrict__ w, const double *__restrict__ d, double *__restrict__ v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v[i] = w[i] / d[i];
+}
+
+template <typename T>
+__global__ void k_recip(int64_t n, const double *__restrict__ d, T *__restrict__ dinv) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dinv[i] = static_cast<T>(1.0 / d[i]);
+}
+
+// ------------------------------------------------------------------- levels
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+struct ptq2_mg {
+  ptq2_mg_opts opt{};
+  int nl = 0;
+  bool f32 = true;
+  hipStream_t stream = nullptr;
+  std::vector<ptq2_ctx *> ctx;          // level contexts (ctx[0] is the caller's)
+  std::vector<bool> owned;              // whether ctx[l] was created here
+  std::vector<int> m;                   // (mx,my,mz) triples
+  std::vector<int64_t> n;               // velocity dofs per level
+  std::vector<void *> vb, vx, vy, vp1, vp2, vdinv;  // level vectors in the smoothing precision
+  std::vector<double> emax, emin;
+  double *ainv = nullptr;               // dense inverse of the coarsest operator
+  double *r_in = nullptr, *z_out = nullptr;  // FP64 graph input/output (level 0)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // Krylov workspace
+  std::vector<double *> Vk, Zk;
+  double *w = nullptr, *part = nullptr, *hdev = nullptr;
+  const double **Vptr = nullptr;  // device array of pointers (Vk)
+  const double **Zptr = nullptr;
+  std::vector<double> hist;
+  std::string last;
+  void *fine_stream0 = nullptr;  // the fine context's stream before the hierarchy was built
+};
+
+namespace {
+
+int mgfail(ptq2_mg *g, int code, const char *msg) {
+  if (g) g->last = msg;
+  return code;
+}
+
+#define MGCHK(call)                                                                    \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return mgfail(mg, PTQ2_ERR_DEVICE, hipGetErrorString(e_));   \
+  } while (0)
+#define MGRC(call)             \
+  do {                         \
+    int r_ = (call);           \
+    if (r_ != PTQ2_OK) return r_; \
+  } while (0)
+
+int mg_alloc(ptq2_mg *mg, void **p, size_t bytes) {
+  if (hipMalloc(p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return mgfail(mg, PTQ2_ERR_ALLOC, "hipMalloc failed in the multigrid set-up");
+  }
+  return PTQ2_OK;
+}
+
+// y = A x on level l in precision T (MatMult semantics: masked columns, y_D = x_D)
+template <typename T>
+int level_matmult(ptq2_mg *mg, int l, const T *x, T *y);
+template <>
+int level_matmult<double>(ptq2_mg *mg, int l, const double *x, double *y) {
+  ptq2_ctx *c = mg->ctx[l];
+  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream));
+  MGCHK(ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
+  return PTQ2_OK;
+}
+template <>
+int level_matmult<float>(ptq2_mg *mg, int l, const float *x, float *y) {
+  ptq2_ctx *c = mg->ctx[l];
+  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream));
+  MGCHK(ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
+  return PTQ2_OK;
+}
+
+// ChebychevRN on level l: x <- smoothed solution of A x = b; zero_guess skips the first residual
+template <typename T>
+int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
+  const int64_t n = mg->n[l];
+  T *b = (T *)mg->vb[l], *x = (T *)mg->vx[l], *y = (T *)mg->vy[l], *dinv = (T *)mg->vdinv[l];
+  T *buf[3] = {x, (T *)mg->vp1[l], (T *)mg->vp2[l]};
+  const double emax = mg->emax[l], emin = mg->emin[l];
+  const double scale = 2.0 / (emax + emin);
+  const double alpha = 1.0 - scale * emin;
+  const double mu = 1.0 / alpha, omegaprod = 2.0 / alpha;
+  double c[3];
+  int km1 = 0, k = 1, kp1 = 2;
+  c[km1] = 1.0;
+  c[k] = mu;
+  if (zero_guess) {
+    hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const T *)b, (const T *)nullptr,
+                       (const T *)dinv, (const T *)nullptr, buf[k], (T)scale);
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, x, T(0));
+  } else {
+    MGRC(level_matmult<T>(mg, l, x, y));
+    hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const T *)b, (const T *)y,
+                       (const T *)dinv, (const T *)x, buf[k], (T)scale);
+  }
+  for (int i = 0; i < mg->opt.smooth_its; i++) {
+    c[kp1] = 2.0 * mu * c[k] - c[km1];
+    const double omega = omegaprod * c[k] / c[kp1];
+    MGRC(level_matmult<T>(mg, l, buf[k], y));
+    hipLaunchKernelGGL(k_cheb_step<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const T *)b, (const T *)y,
+                       (const T *)dinv, (const T *)buf[km1], (const T *)buf[k], buf[kp1], (T)(omega * scale),
+                       (T)(1.0 - omega), (T)omega);
+    int t = km1;
+    km1 = k;
+    k = kp1;
+    kp1 = t;
+  }
+  if (k != 0) MGCHK(hipMemcpyAsync(x, buf[k], sizeof(T) * n, hipMemcpyDeviceToDevice, mg->stream));
+  return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "chebyshev launch");
+}
+
+template <typename T>
+int vcycle_level(ptq2_mg *mg, int l) {
+  const int64_t n = mg->n[l];
+  T *b = (T *)mg->vb[l], *x = (T *)mg->vx[l], *y = (T *)mg->vy[l];
+  if (l == mg->nl - 1) {
+    int nn = (int)n;
+    hipLaunchKernelGGL(k_dense_gemv<T>, dim3((nn + 3) / 4), dim3(256), 0, mg->stream, nn, (const double *)mg->ainv,
+                       (const T *)b, x);
+    return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "coarse solve launch");
+  }
+  MGRC(chebyshev<T>(mg, l, true));
+  MGRC(level_matmult<T>(mg, l, x, y));
+  const int *mf = &mg->m[3 * l], *mc = &mg->m[3 * (l + 1)];
+  const int NXf = 2 * mf[0] + 1, NYf = 2 * mf[1] + 1, NZf = 2 * mf[2] + 1;
+  const int NXc = 2 * mc[0] + 1, NYc = 2 * mc[1] + 1, NZc = 2 * mc[2] + 1;
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(mg->n[l + 1])), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, NXf, NYf, NZf,
+                     (const T *)b, (const T *)y, (const uint8_t *)mg->ctx[l + 1]->mask, (T *)mg->vb[l + 1]);
+  MGRC(vcycle_level<T>(mg, l + 1));
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, NXc, NYc, NXf, NYf, NZf,
+                     (const T *)mg->vx[l + 1], (const uint8_t *)mg->ctx[l]->mask, x);
+  MGRC(chebyshev<T>(mg, l, false));
+  return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle launch");
+}
+
+// the work of one V-cycle: r_in (FP64) -> z_out (FP64)
+int vcycle_body(ptq2_mg *mg) {
+  const int64_t n = mg->n[0];
+  if (mg->f32) {
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)mg->r_in,
+                       (float *)mg->vb[0]);
+    MGRC(vcycle_level<float>(mg, 0));
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const float *)mg->vx[0],
+                       mg->z_out);
+  } else {
+    MGCHK(hipMemcpyAsync(mg->vb[0], mg->r_in, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+    MGRC(vcycle_level<double>(mg, 0));
+    MGCHK(hipMemcpyAsync(mg->z_out, mg->vx[0], sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  }
+  return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle");
+}
+
+// deterministic dot products: per-block partials summed on the host in a fixed order
+int mdot(ptq2_mg *mg, const double **Vdev, int nv, const double *w, int64_t n, double *out) {
+  const unsigned nb = 1024;
+  hipLaunchKernelGGL(k_mdot, dim3(nb), dim3(TPB), 0, mg->stream, n, Vdev, nv, w, mg->part);
+  std::vector<double> host((size_t)nv * nb);
+  MGCHK(hipMemcpyAsync(host.data(), mg->part, sizeof(double) * nv * nb, hipMemcpyDeviceToHost, mg->stream));
+  MGCHK(hipStreamSynchronize(mg->stream));
+  for (int v = 0; v < nv; v++) {
+    double s = 0.0;
+    for (unsigned b = 0; b < nb; b++) s += host[(size_t)v * nb + b];
+    out[v] = s;
+  }
+  return PTQ2_OK;
+}
+
+int norm2(ptq2_mg *mg, const double *x, int64_t n, double *out) {
+  // a pointer table entry pointing at x itself, stored after the Krylov tables
+  const double **slot = mg->Zptr + 64;
+  MGCHK(hipMemcpyAsync((void *)slot, &x, sizeof(double *), hipMemcpyHostToDevice, mg->stream));
+  double d;
+  MGRC(mdot(mg, slot, 1, x, n, &d));
+  *out = std::sqrt(std::max(d, 0.0));
+  return PTQ2_OK;
+}
+
+// dense Cholesky inverse of an SPD matrix (row-major, n x n), in place
+bool spd_inverse(std::vector<double> &A, int n) {
+  std::vector<double> L(A);
+  for (int j = 0; j < n; j++) {
+    double *Lj = &L[(size_t)j * n];
+    double d = Lj[j];
+    for (int k = 0; k < j; k++) d -= Lj[k] * Lj[k];
+    if (!(d > 0.0)) return false;
+    d = std::sqrt(d);
+    Lj[j] = d;
+    for (int i = j + 1; i < n; i++) {
+      double *Li = &L[(size_t)i * n];
+      double s = Li[j];
+      for (int k = 0; k < j; k++) s -= Li[k] * Lj[k];
+      Li[j] = s / d;
+    }
+  }
+  // inverse: solve L L^T X = I column by column
+  std::vector<double> col(n);
+  for (int c = 0; c < n; c++) {
+    std::fill(col.begin(), col.end(), 0.0);
+    col[c] = 1.0;
+    for (int i = 0; i < n; i++) {
+      const double *Li = &L[(size_t)i * n];
+      double s = col[i];
+      for (int k = 0; k < i; k++) s -= Li[k] * col[k];
+      col[i] = s / Li[i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double s = col[i];
+      for (int k = i + 1; k < n; k++) s -= L[(size_t)k * n + i] * col[k];
+      col[i] = s / L[(size_t)i * n + i];
+    }
+    for (int i = 0; i < n; i++) A[(size_t)i * n + c] = col[i];
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+void ptq2_mg_default_opts(ptq2_mg_opts *o) {
+  if (!o) return;
+  o->nlevels = 5;
+  o->smooth_its = 4;
+  o->fp32_smoother = 1;
+  o->emin_frac = 0.2;
+  o->emax_frac = 1.1;
+  o->esteig_its = 20;
+  o->use_graph = 1;
+  o->coarse_max_dofs = 6000;
+}
+
+int ptq2_mg_destroy(ptq2_mg **pmg) {
+  if (!pmg || !*pmg) return PTQ2_OK;
+  ptq2_mg *mg = *pmg;
+  if (mg->stream) (void)hipStreamSynchronize(mg->stream);
+  if (mg->exec) (void)hipGraphExecDestroy(mg->exec);
+  if (mg->graph) (void)hipGraphDestroy(mg->graph);
+  for (size_t l = 0; l < mg->vb.size(); l++) {
+    for (void *p : {mg->vb[l], mg->vx[l], mg->vy[l], mg->vp1[l], mg->vp2[l], mg->vdinv[l]})
+      if (p) (void)hipFree(p);
+  }
+  for (size_t l = 1; l < mg->ctx.size(); l++)
+    if (mg->owned[l]) ptq2_destroy(&mg->ctx[l]);
+  if (!mg->ctx.empty() && mg->ctx[0]) ptq2_set_stream(mg->ctx[0], mg->fine_stream0);
+  for (double *p : mg->Vk) (void)hipFree(p);
+  for (double *p : mg->Zk) (void)hipFree(p);
+  for (void *p : {(void *)mg->ainv, (void *)mg->r_in, (void *)mg->z_out, (void *)mg->w, (void *)mg->part,
+                  (void *)mg->hdev, (void *)mg->Vptr, (void *)mg->Zptr})
+    if (p) (void)hipFree(p);
+  delete mg;
+  *pmg = nullptr;
+  return PTQ2_OK;
+}
+
+int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
+  if (!out || !fine) return PTQ2_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (!fine->mesh_ready || !fine->eta_ready) return PTQ2_ERR_NOT_SETUP;
+  if (fine->box.i0 || fine->box.j0 || fine->box.k0 || fine->box.NX != 2 * fine->box.mx + 1 ||
+      fine->box.NY != 2 * fine->box.my + 1 || fine->box.NZ != 2 * fine->box.mz + 1)
+    return PTQ2_ERR_UNSUPPORTED;
+  ptq2_mg_opts o;
+  if (opts) o = *opts;
+  else ptq2_mg_default_opts(&o);
+  if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0))
+    return PTQ2_ERR_INVALID_ARG;
+  for (int l = 1; l < o.nlevels; l++) {
+    int f = 1 << l;
+    if (fine->box.mx % f || fine->box.my % f || fine->box.mz % f) return PTQ2_ERR_INVALID_ARG;
+  }
+  ptq2_mg *mg = new (std::nothrow) ptq2_mg();
+  if (!mg) return PTQ2_ERR_ALLOC;
+  std::unique_ptr<ptq2_mg, int (*)(ptq2_mg *)> guard(mg, [](ptq2_mg *p) { return ptq2_mg_destroy(&p); });
+  mg->opt = o;
+  mg->nl = o.nlevels;
+  mg->f32 = o.fp32_smoother != 0;
+  // set-up and graph capture run on the fine context's own (capturable, non-null) stream
+  mg->fine_stream0 = (void *)fine->stream;
+  mg->stream = fine->own_stream;
+  ptq2_set_stream(fine, (void *)mg->stream);
+  MGCHK(hipSetDevice(fine->device));
+  MGCHK(hipStreamSynchronize((hipStream_t)mg->fine_stream0));
+
+  // ---- level meshes
+  mg->ctx.push_back(fine);
+  mg->owned.push_back(false);
+  mg->m = {fine->box.mx, fine->box.my, fine->box.mz};
+  mg->n.push_back(fine->nvel);
+  std::vector<double> cf((size_t)fine->nvel);
+  std::vector<uint8_t> mf((size_t)fine->nvel, 0);
+  MGCHK(hipMemcpy(cf.data(), fine->coords, sizeof(double) * fine->nvel, hipMemcpyDeviceToHost));
+  if (fine->mask) MGCHK(hipMemcpy(mf.data(), fine->mask, fine->nvel, hipMemcpyDeviceToHost));
+  for (int l = 1; l < mg->nl; l++) {
+    const int *pm = &mg->m[3 * (l - 1)];
+    int mx = pm[0] / 2, my = pm[1] / 2, mz = pm[2] / 2;
+    int NXf = 2 * pm[0] + 1, NYf = 2 * pm[1] + 1;
+    int NX = 2 * mx + 1, NY = 2 * my + 1, NZ = 2 * mz + 1;
+    std::vector<double> cc((size_t)3 * NX * NY * NZ);
+    std::vector<uint8_t> mc((size_t)3 * NX * NY * NZ);
+    for (int k = 0; k < NZ; k++)
+      for (int j = 0; j < NY; j++)
+        for (int i = 0; i < NX; i++) {
+          int64_t a = (int64_t)i + (int64_t)NX * (j + (int64_t)NY * k);
+          int64_t b = (int64_t)(2 * i) + (int64_t)NXf * (2 * j + (int64_t)NYf * (2 * k));
+          for (int d = 0; d < 3; d++) {
+            cc[3 * a + d] = cf[3 * b + d];
+            mc[3 * a + d] = mf[3 * b + d];
+          }
+        }
+    ptq2_ctx *c = nullptr;
+    MGRC(ptq2_create(&c, fine->device));
+    mg->ctx.push_back(c);
+    mg->owned.push_back(true);
+    ptq2_set_stream(c, mg->stream);
+    MGRC(ptq2_setup_mesh(c, mx, my, mz, cc.data(), nullptr, fine->mask ? mc.data() : nullptr));
+    // viscosity from the finer level
+    double *etac = nullptr;
+    MGRC(mg_alloc(mg, (void **)&etac, sizeof(double) * 27 * (size_t)c->nel));
+    ptq2_ctx *p = mg->ctx[l - 1];
+    hipLaunchKernelGGL(k_eta_coarsen, dim3(nblk(27 * c->nel)), dim3(TPB), 0, mg->stream, p->box, mx, my, mz,
+                       (const double *)p->eta_bm, etac);
+    int rc = ptq2_set_viscosity(c, etac, 0, PTQ2_DEVICE);
+    (void)hipFree(etac);
+    MGRC(rc);
+    mg->m.insert(mg->m.end(), {mx, my, mz});
+    mg->n.push_back(c->nvel);
+    cf.swap(cc);
+    mf.swap(mc);
+  }
+  if (mg->n.back() > o.coarse_max_dofs) return mgfail(mg, PTQ2_ERR_SIZE, "coarse level too large for the dense solve");
+
+  // ---- small reduction buffers (also used by the solve)
+  MGRC(mg_alloc(mg, (void **)&mg->part, sizeof(double) * 64 * 1024));
+  MGRC(mg_alloc(mg, (void **)&mg->hdev, sizeof(double) * 64));
+  MGRC(mg_alloc(mg, (void **)&mg->Vptr, sizeof(double *) * 64));
+  MGRC(mg_alloc(mg, (void **)&mg->Zptr, sizeof(double *) * 65));
+
+  // ---- level vectors, Jacobi diagonals, eigen-estimates
+  const size_t es = mg->f32 ? sizeof(float) : sizeof(double);
+  mg->vb.assign(mg->nl, nullptr); mg->vx.assign(mg->nl, nullptr); mg->vy.assign(mg->nl, nullptr);
+  mg->vp1.assign(mg->nl, nullptr); mg->vp2.assign(mg->nl, nullptr); mg->vdinv.assign(mg->nl, nullptr);
+  mg->emax.assign(mg->nl, 0.0); mg->emin.assign(mg->nl, 0.0);
+  for (int l = 0; l < mg->nl; l++) {
+    const int64_t n = mg->n[l];
+    MGRC(mg_alloc(mg, &mg->vb[l], es * n));
+    MGRC(mg_alloc(mg, &mg->vx[l], es * n));
+    if (l == mg->nl - 1) break;
+    MGRC(mg_alloc(mg, &mg->vy[l], es * n));
+    MGRC(mg_alloc(mg, &mg->vp1[l], es * n));
+    MGRC(mg_alloc(mg, &mg->vp2[l], es * n));
+    MGRC(mg_alloc(mg, &mg->vdinv[l], es * n));
+    ptq2_ctx *c = mg->ctx[l];
+    if (mg->f32) MGRC(ptq2_internal_build_fac32(c));
+    double *d = nullptr, *v = nullptr, *w = nullptr;
+    MGRC(mg_alloc(mg, (void **)&d, sizeof(double) * n));
+    MGRC(mg_alloc(mg, (void **)&v, sizeof(double) * n));
+    MGRC(mg_alloc(mg, (void **)&w, sizeof(double) * n));
+    int rc = ptq2_a11_diag(c, d, PTQ2_DEVICE | PTQ2_MATMULT);
+    if (rc == PTQ2_OK) {
+      if (mg->f32) hipLaunchKernelGGL(k_recip<float>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)d, (float *)mg->vdinv[l]);
+      else hipLaunchKernelGGL(k_recip<double>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)d, (double *)mg->vdinv[l]);
+      // power iteration on D^-1 A (FP64, MatMult semantics) with the Rayleigh quotient
+      // v^T A v / v^T D v; the start vector is a SplitMix64 stream
+      std::vector<double> hv((size_t)n);
+      uint64_t s = 0x9E3779B97F4A7C15ull ^ (uint64_t)l;
+      for (int64_t i = 0; i < n; i++) {
+        s += 0x9E3779B97F4A7C15ull;
+        uint64_t z = s;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        hv[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+      }
+      if (hipMemcpy(v, hv.data(), sizeof(double) * n, hipMemcpyHostToDevice) != hipSuccess) rc = PTQ2_ERR_DEVICE;
+      double lam = 0.0;
+      const unsigned nb = 1024;
+      std::vector<double> hp(2 * nb);
+      for (int it = 0; it < o.esteig_its && rc == PTQ2_OK; it++) {
+        rc = ptq2_a11_apply(c, v, w, PTQ2_DEVICE | PTQ2_MATMULT);
+        if (rc) break;
+        hipLaunchKernelGGL(k_rq_parts, dim3(nb), dim3(TPB), 0, mg->stream, n, (const double *)v, (const double *)w,
+                           (const double *)d, mg->part);
+        hipLaunchKernelGGL(k_div, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)w, (const double *)d, v);
+        if (hipMemcpyAsync(hp.data(), mg->part, sizeof(double) * 2 * nb, hipMemcpyDeviceToHost, mg->stream) != hipSuccess ||
+            hipStreamSynchronize(mg->stream) != hipSuccess) {
+          rc = PTQ2_ERR_DEVICE;
+          break;
+        }
+        double vav = 0.0, vdv = 0.0;
+        for (unsigned k = 0; k < nb; k++) {
+          vav += hp[2 * k];
+          vdv += hp[2 * k + 1];
+        }
+        lam = vav / vdv;
+        double nrm;
+        rc = norm2(mg, v, n, &nrm);
+        if (rc) break;
+        hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)v, 1.0 / nrm, v);
+      }
+      mg->emax[l] = o.emax_frac * lam;
+      mg->emin[l] = o.emin_frac * lam;
+    }
+    (void)hipFree(d); (void)hipFree(v); (void)hipFree(w);
+    MGRC(rc);
+  }
+  // ---- dense coarse inverse
+  {
+    const int l = mg->nl - 1;
+    ptq2_ctx *c = mg->ctx[l];
+    const int nc = (int)mg->n[l];
+    std::vector<double> A((size_t)nc * nc);
+    double *e = nullptr, *y = nullptr;
+    MGRC(mg_alloc(mg, (void **)&e, sizeof(double) * nc));
+    MGRC(mg_alloc(mg, (void **)&y, sizeof(double) * nc));
+    std::vector<double> col((size_t)nc);
+    int rc = PTQ2_OK;
+    for (int j = 0; j < nc && rc == PTQ2_OK; j++) {
+      std::fill(col.begin(), col.end(), 0.0);
+      col[j] = 1.0;
+      (void)hipMemcpyAsync(e, col.data(), sizeof(double) * nc, hipMemcpyHostToDevice, mg->stream);
+      rc = ptq2_a11_apply(c, e, y, PTQ2_DEVICE | PTQ2_MATMULT);
+      (void)hipMemcpyAsync(col.data(), y, sizeof(double) * nc, hipMemcpyDeviceToHost, mg->stream);
+      (void)hipStreamSynchronize(mg->stream);
+      for (int i = 0; i < nc; i++) A[(size_t)i * nc + j] = col[i];
+    }
+    (void)hipFree(e); (void)hipFree(y);
+    MGRC(rc);
+    // symmetrise the round-off before factorising
+    for (int i = 0; i < nc; i++)
+      for (int j = i + 1; j < nc; j++) {
+        double s = 0.5 * (A[(size_t)i * nc + j] + A[(size_t)j * nc + i]);
+        A[(size_t)i * nc + j] = A[(size_t)j * nc + i] = s;
+      }
+    if (!spd_inverse(A, nc)) return mgfail(mg, PTQ2_ERR_MESH, "coarse operator is not positive definite");
+    MGRC(mg_alloc(mg, (void **)&mg->ainv, sizeof(double) * (size_t)nc * nc));
+    MGCHK(hipMemcpy(mg->ainv, A.data(), sizeof(double) * (size_t)nc * nc, hipMemcpyHostToDevice));
+  }
+  // ---- graph of one V-cycle
+  MGRC(mg_alloc(mg, (void **)&mg->r_in, sizeof(double) * mg->n[0]));
+  MGRC(mg_alloc(mg, (void **)&mg->z_out, sizeof(double) * mg->n[0]));
+  if (o.use_graph) {
+    MGCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
+    int rc = vcycle_body(mg);
+    hipGraph_t gph = nullptr;
+    hipError_t e = hipStreamEndCapture(mg->stream, &gph);
+    if (rc) {
+      if (gph) (void)hipGraphDestroy(gph);
+      return rc;
+    }
+    if (e != hipSuccess) return mgfail(mg, PTQ2_ERR_DEVICE, hipGetErrorString(e));
+    mg->graph = gph;
+    MGCHK(hipGraphInstantiate(&mg->exec, mg->graph, nullptr, nullptr, 0));
+  }
+  MGCHK(hipStreamSynchronize(mg->stream));
+  guard.release();
+  *out = mg;
+  return PTQ2_OK;
+}
+
+int ptq2_mg_levels(const ptq2_mg *mg, int *nl, int *m) {
+  if (!mg || !nl) return PTQ2_ERR_INVALID_ARG;
+  *nl = mg->nl;
+  if (m) std::memcpy(m, mg->m.data(), sizeof(int) * mg->m.size());
+  return PTQ2_OK;
+}
+
+int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax) {
+  if (!mg) return PTQ2_ERR_INVALID_ARG;
+  for (int l = 0; l + 1 < mg->nl; l++) {
+    if (emin) emin[l] = mg->emin[l];
+    if (emax) emax[l] = mg->emax[l];
+  }
+  return PTQ2_OK;
+}
+
+int ptq2_mg_set_stream(ptq2_mg *mg, void *s) {
+  if (!mg) return PTQ2_ERR_INVALID_ARG;
+  mg->stream = (hipStream_t)s;
+  for (ptq2_ctx *c : mg->ctx) ptq2_set_stream(c, s);
+  return PTQ2_OK;
+}
+
+int ptq2_mg_vcycle(ptq2_mg *mg, const double *r, double *z) {
+  if (!mg || !r || !z) return PTQ2_ERR_INVALID_ARG;
+  const int64_t n = mg->n[0];
+  MGCHK(hipMemcpyAsync(mg->r_in, r, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  if (mg->exec) MGCHK(hipGraphLaunch(mg->exec, mg->stream));
+  else MGRC(vcycle_body(mg));
+  MGCHK(hipMemcpyAsync(z, mg->z_out, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  return PTQ2_OK;
+}
+
+int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_it, int restart, int *iters,
+                  double *final_rnorm, double *rnorm0) {
+  if (!mg || !b || !x || restart < 1 || restart > 60 || max_it < 1) return PTQ2_ERR_INVALID_ARG;
+  const int64_t n = mg->n[0];
+  ptq2_ctx *fc = mg->ctx[0];
+  // workspace
+  if ((int)mg->Vk.size() < restart + 1) {
+    while ((int)mg->Vk.size() < restart + 1) {
+      double *p = nullptr;
+      MGRC(mg_alloc(mg, (void **)&p, sizeof(double) * n));
+      mg->Vk.push_back(p);
+    }
+    while ((int)mg->Zk.size() < restart) {
+      double *p = nullptr;
+      MGRC(mg_alloc(mg, (void **)&p, sizeof(double) * n));
+      mg->Zk.push_back(p);
+    }
+  }
+  if (!mg->w) MGRC(mg_alloc(mg, (void **)&mg->w, sizeof(double) * n));
+  std::vector<const double *> vp(mg->Vk.begin(), mg->Vk.end()), zp(mg->Zk.begin(), mg->Zk.end());
+  MGCHK(hipMemcpy((void *)mg->Vptr, vp.data(), sizeof(double *) * vp.size(), hipMemcpyHostToDevice));
+  MGCHK(hipMemcpy((void *)mg->Zptr, zp.data(), sizeof(double *) * zp.size(), hipMemcpyHostToDevice));
+  ptq2_set_stream(fc, mg->stream);
+  mg->hist.clear();
+  const unsigned gb = nblk(n);
+  hipLaunchKernelGGL(k_fill<double>, dim3(gb), dim3(TPB), 0, mg->stream, n, x, 0.0);
+  double bnorm = 0.0;
+  MGRC(norm2(mg, b, n, &bnorm));
+  if (rnorm0) *rnorm0 = bnorm;
+  int it = 0;
+  double rn = bnorm;
+  mg->hist.push_back(rn);
+  if (bnorm == 0.0) {
+    if (iters) *iters = 0;
+    if (final_rnorm) *final_rnorm = 0.0;
+    return PTQ2_OK;
+  }
+  const double target = rtol * bnorm;
+  std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), h(restart + 1),
+      h2(restart + 1), y(restart);
+  while (it < max_it) {
+    // r = b - A x
+    MGRC(ptq2_a11_apply(fc, x, mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
+    hipLaunchKernelGGL(k_sub, dim3(gb), dim3(TPB), 0, mg->stream, n, b, (const double *)mg->w, mg->Vk[0]);
+    double beta;
+    MGRC(norm2(mg, mg->Vk[0], n, &beta));
+    if (beta <= target) {
+      rn = beta;
+      break;
+    }
+    hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->Vk[0], 1.0 / beta, mg->Vk[0]);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int j = 0;
+    for (; j < restart && it < max_it; j++) {
+      MGRC(ptq2_mg_vcycle(mg, mg->Vk[j], mg->Zk[j]));
+      MGRC(ptq2_a11_apply(fc, mg->Zk[j], mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
+      // classical Gram-Schmidt with one re-orthogonalisation pass
+      MGRC(mdot(mg, mg->Vptr, j + 1, mg->w, n, h.data()));
+      MGCHK(hipMemcpyAsync(mg->hdev, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
+      hipLaunchKernelGGL(k_maxpy, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Vptr, j + 1, (const double *)mg->hdev, mg->w);
+      MGRC(mdot(mg, mg->Vptr, j + 1, mg->w, n, h2.data()));
+      MGCHK(hipMemcpyAsync(mg->hdev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
+      hipLaunchKernelGGL(k_maxpy, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Vptr, j + 1, (const double *)mg->hdev, mg->w);
+      for (int i = 0; i <= j; i++) h[i] += h2[i];
+      double hn;
+      MGRC(norm2(mg, mg->w, n, &hn));
+      h[j + 1] = hn;
+      if (hn > 0.0)
+        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->w, 1.0 / hn, mg->Vk[j + 1]);
+      // Givens rotations
+      for (int i = 0; i < j; i++) {
+        double t = cs[i] * h[i] + sn[i] * h[i + 1];
+        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
+        h[i] = t;
+      }
+      double den = std::hypot(h[j], h[j + 1]);
+      cs[j] = den > 0 ? h[j] / den : 1.0;
+      sn[j] = den > 0 ? h[j + 1] / den : 0.0;
+      h[j] = den;
+      h[j + 1] = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      for (int i = 0; i <= j; i++) H[(size_t)i * restart + j] = h[i];
+      it++;
+      rn = std::fabs(g[j + 1]);
+      mg->hist.push_back(rn);
+      if (rn <= target || hn == 0.0) {
+        j++;
+        break;
+      }
+    }
+    // x += Z y, with H(0:j,0:j) y = g
+    for (int i = j - 1; i >= 0; i--) {
+      double s = g[i];
+      for (int k = i + 1; k < j; k++) s -= H[(size_t)i * restart + k] * y[k];
+      y[i] = s / H[(size_t)i * restart + i];
+    }
+    MGCHK(hipMemcpyAsync(mg->hdev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, mg->stream));
+    hipLaunchKernelGGL(k_mcombine, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Zptr, j, (const double *)mg->hdev, x);
+    if (rn <= target) break;
+  }
+  // true residual
+  MGRC(ptq2_a11_apply(fc, x, mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
+  hipLaunchKernelGGL(k_sub, dim3(gb), dim3(TPB), 0, mg->stream, n, b, (const double *)mg->w, mg->w);
+  double tr;
+  MGRC(norm2(mg, mg->w, n, &tr));
+  if (iters) *iters = it;
+  if (final_rnorm) *final_rnorm = tr;
+  return PTQ2_OK;
+}
+
+int ptq2_mg_history(const ptq2_mg *mg, double *hist, int cap, int *n) {
+  if (!mg || !n) return PTQ2_ERR_INVALID_ARG;
+  int k = (int)mg->hist.size();
+  *n = k;
+  if (hist)
+    for (int i = 0; i < std::min(k, cap); i++) hist[i] = mg->hist[i];
+  return PTQ2_OK;
+}
+
+}  // extern "C"

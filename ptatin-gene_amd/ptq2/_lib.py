@@ -33,6 +33,7 @@ SIGNATURES = {
     "ptq2_strerror": (ctypes.c_char_p, [_c_int]),
     "ptq2_last_error": (ctypes.c_char_p, [_vp]),
     "ptq2_set_stream": (_c_int, [_vp, _vp]),
+    "ptq2_get_stream": (_vp, [_vp]),
     "ptq2_sync": (_c_int, [_vp]),
     "ptq2_setup_mesh": (_c_int, [_vp, _c_int, _c_int, _c_int, _vp, _vp, _vp]),
     "ptq2_setup_mesh_box": (_c_int, [_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int,

@@ -57,9 +57,10 @@ class Context:
             pass
 
     def _stream(self, *tensors):
+        """Order the library's work on torch's current stream (the null stream by default)."""
         if any(_is_dev(t) for t in tensors):
             s = torch.cuda.current_stream(device=tensors[0].device).cuda_stream
-            _lib.lib().ptq2_set_stream(self.handle, ctypes.c_void_p(s))
+            _lib.lib().ptq2_set_stream(self.handle, ctypes.c_void_p(s) if s else None)
 
     def setup_mesh(self, mx, my, mz, coords, mask=None, elnidx=None):
         coords = np.ascontiguousarray(coords, dtype=np.float64)

@@ -1,0 +1,129 @@
+"""Host wrapper of the GMG-preconditioned FGMRES viscous solve (include/ptq2_solver.h).
+
+Mirrors the PETSc configuration pTatin3D builds for the velocity block
+(pTatin3dStokesKSPConfigureFSGMG, src/ptatin_driver_linear_ts.c:820-883, with the
+sinker options of tests/application/linear/sinker_mf_8x8x8/opts): PCMG, Chebyshev +
+Jacobi smoothers with the (0, 0.2, 0, 1.1) eigen-estimate transform, FGMRES outside.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check
+
+
+class MGOpts(ctypes.Structure):
+    _fields_ = [("nlevels", ctypes.c_int), ("smooth_its", ctypes.c_int), ("fp32_smoother", ctypes.c_int),
+                ("emin_frac", ctypes.c_double), ("emax_frac", ctypes.c_double), ("esteig_its", ctypes.c_int),
+                ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int)]
+
+
+_SOLVER_SIGS = {
+    "ptq2_mg_default_opts": (None, [ctypes.POINTER(MGOpts)]),
+    "ptq2_mg_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_void_p, ctypes.POINTER(MGOpts)]),
+    "ptq2_mg_destroy": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "ptq2_mg_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
+    "ptq2_mg_eigs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "ptq2_mg_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "ptq2_mg_vcycle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "ptq2_mg_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double)]),
+    "ptq2_mg_history": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+}
+_bound = False
+
+
+def _sl():
+    global _bound
+    L = _lib.lib()
+    if not _bound:
+        for name, (res, args) in _SOLVER_SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _bound = True
+    return L
+
+
+def default_opts():
+    o = MGOpts()
+    _sl().ptq2_mg_default_opts(ctypes.byref(o))
+    return o
+
+
+class ViscousSolver:
+    """GMG V-cycle (hipGraph) + FGMRES for MatMult_MFStokes_A11 x = b on one GPU."""
+
+    def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=6000):
+        L = _sl()
+        o = default_opts()
+        o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
+        o.esteig_its, o.use_graph, o.coarse_max_dofs = esteig_its, 1 if use_graph else 0, coarse_max_dofs
+        self.ctx = ctx  # keep the fine context alive
+        h = ctypes.c_void_p()
+        check(L.ptq2_mg_create(ctypes.byref(h), ctx.handle, ctypes.byref(o)), "ptq2_mg_create", ctx.handle)
+        self._h = h
+        self.nlevels = nlevels
+        self.fp32 = fp32
+
+    def levels(self):
+        n = ctypes.c_int()
+        m = np.zeros(3 * self.nlevels, dtype=np.int32)
+        check(_sl().ptq2_mg_levels(self._h, ctypes.byref(n), m.ctypes.data), "ptq2_mg_levels")
+        return m.reshape(-1, 3)
+
+    def eigs(self):
+        lo = np.zeros(self.nlevels - 1)
+        hi = np.zeros(self.nlevels - 1)
+        check(_sl().ptq2_mg_eigs(self._h, lo.ctypes.data, hi.ctypes.data), "ptq2_mg_eigs")
+        return lo, hi
+
+    def _stream(self, t):
+        """Order the solver on torch's current stream (the null stream by default)."""
+        import torch
+
+        s = torch.cuda.current_stream(device=t.device).cuda_stream
+        check(_sl().ptq2_mg_set_stream(self._h, ctypes.c_void_p(s) if s else None), "ptq2_mg_set_stream")
+
+    def vcycle(self, r, z=None):
+        import torch
+
+        if z is None:
+            z = torch.empty_like(r)
+        self._stream(r)
+        check(_sl().ptq2_mg_vcycle(self._h, r.data_ptr(), z.data_ptr()), "ptq2_mg_vcycle")
+        return z
+
+    def solve(self, b, x=None, rtol=1e-8, max_it=200, restart=30):
+        import torch
+
+        if x is None:
+            x = torch.empty_like(b)
+        self._stream(b)
+        it = ctypes.c_int()
+        fr = ctypes.c_double()
+        r0 = ctypes.c_double()
+        check(_sl().ptq2_mg_solve(self._h, b.data_ptr(), x.data_ptr(), rtol, max_it, restart, ctypes.byref(it),
+                                  ctypes.byref(fr), ctypes.byref(r0)), "ptq2_mg_solve")
+        return x, {"iterations": it.value, "final_rnorm": fr.value, "rnorm0": r0.value,
+                   "rel": fr.value / r0.value if r0.value else 0.0}
+
+    def history(self):
+        n = ctypes.c_int()
+        _sl().ptq2_mg_history(self._h, None, 0, ctypes.byref(n))
+        h = np.zeros(n.value)
+        _sl().ptq2_mg_history(self._h, h.ctypes.data, n.value, ctypes.byref(n))
+        return h
+
+    def destroy(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _sl().ptq2_mg_destroy(ctypes.byref(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

@@ -1,0 +1,93 @@
+"""GPU multigrid / FGMRES viscous solve against the CPU restatement (oracle/mg_oracle.py)."""
+import numpy as np
+import pytest
+
+from oracle import mg_oracle as G
+from oracle import oracle as O
+import ptq2
+from ptq2 import mesh as M
+from ptq2.solver import ViscousSolver
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+DEV = torch.device("cuda:0")
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def make(m, contrast, bc="freeslip", n_incl=3):
+    P = M.config_sinker(m, n_incl=n_incl, contrast=contrast, bc=bc)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(m, m, m, P.coords, P.mask)
+    ctx.set_viscosity(P.eta)
+    return P, ctx
+
+
+def test_vcycle_fp64_matches_restatement():
+    m = 8
+    say("case: FP64 V-cycle 8^3 (3 levels) vs CPU restatement")
+    P, ctx = make(m, 1e2)
+    S = ViscousSolver(ctx, nlevels=3, smooth_its=4, fp32=False, use_graph=True)
+    assert [tuple(r) for r in S.levels()] == [(8, 8, 8), (4, 4, 4), (2, 2, 2)]
+    emin, emax = S.eigs()
+    lv = G.hierarchy(m, m, m, P.coords, P.mask, P.eta, 3)
+    V = G.VCycle(lv, emin, emax, 4)
+    r = M.uniform(3, P.nvel, -1, 1)
+    r[P.mask > 0] = 0.0
+    z = S.vcycle(torch.from_numpy(r).to(DEV)).cpu().numpy()
+    zr = V(r)
+    assert np.linalg.norm(z - zr) / np.linalg.norm(zr) <= 1e-10
+    # the eigen-estimate is close to the true lambda_max(D^-1 A) of the fine level
+    L = lv[0]
+    d = L.diag()
+    v = M.uniform(4, L.n, -1, 1)
+    for _ in range(80):
+        w = L.matmult(v) / d
+        lam = np.linalg.norm(w) / np.linalg.norm(v)
+        v = w / np.linalg.norm(w)
+    assert 0.9 * lam <= emax[0] / 1.1 <= 1.01 * lam
+
+
+def test_graph_replays_match_eager():
+    m = 8
+    say("case: V-cycle graph replays vs eager launches")
+    P, ctx = make(m, 1e3)
+    Sg = ViscousSolver(ctx, nlevels=3, fp32=True, use_graph=True)
+    P2, ctx2 = make(m, 1e3)
+    Se = ViscousSolver(ctx2, nlevels=3, fp32=True, use_graph=False)
+    for k in range(4):
+        r = torch.from_numpy(M.uniform(10 + k, P.nvel, -1, 1)).to(DEV)
+        zg = Sg.vcycle(r).clone()
+        _ = torch.ones(1000, device=DEV).sum().item()  # eager work and a sync between replays
+        ze = Se.vcycle(r)
+        torch.cuda.synchronize()
+        assert torch.equal(zg, ze), k
+
+
+def test_fgmres_solve_16():
+    """FGMRES(30) + GMG to rtol 1e-8 at viscosity contrast 1e4, FP64 and FP32 smoothers.
+
+    Both reach the FP64 true residual; the FP32 smoother costs at most a few extra
+    iterations (its error is far below the Chebyshev smoothing factor)."""
+    m = 16
+    P, ctx = make(m, 1e4, n_incl=6)
+    xs = M.uniform(21, P.nvel, -1, 1)
+    xs[P.mask > 0] = 0.0
+    b = O.matmult_a11(m, m, m, P.coords, P.eta, P.mask, xs)
+    its = {}
+    for fp32 in (False, True):
+        say("case: FGMRES + GMG solve 16^3, contrast 1e4, fp32 smoother=%s" % fp32)
+        S = ViscousSolver(ctx, nlevels=4, smooth_its=4, fp32=fp32)
+        x, info = S.solve(torch.from_numpy(b).to(DEV), rtol=1e-8, max_it=400, restart=30)
+        x = x.cpu().numpy()
+        say("  iterations %d, rel residual %.3e" % (info["iterations"], info["rel"]))
+        r = b - O.matmult_a11(m, m, m, P.coords, P.eta, P.mask, x)
+        assert np.linalg.norm(r) <= 1.05e-8 * np.linalg.norm(b)
+        h = S.history()
+        assert h[0] == pytest.approx(np.linalg.norm(b), rel=1e-12)
+        its[fp32] = info["iterations"]
+        S.destroy()
+    assert its[True] <= its[False] + max(3, its[False] // 10)

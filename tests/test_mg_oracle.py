@@ -1,0 +1,101 @@
+"""Pins of the CPU multigrid restatement (oracle/mg_oracle.py) -- no GPU."""
+import numpy as np
+
+from oracle import mg_oracle as G
+from ptq2 import mesh as M
+
+
+def test_restriction_is_prolongation_transpose():
+    mxc, myc, mzc = 2, 3, 2
+    nc = 3 * (2 * mxc + 1) * (2 * myc + 1) * (2 * mzc + 1)
+    nf = 3 * (4 * mxc + 1) * (4 * myc + 1) * (4 * mzc + 1)
+    a = M.uniform(1, nc, -1, 1)
+    b = M.uniform(2, nf, -1, 1)
+    assert abs(G.prolong(mxc, myc, mzc, a) @ b - a @ G.restrict(mxc, myc, mzc, b)) < 1e-13
+
+
+def test_prolongation_reproduces_trilinear_fields():
+    m = 4
+    fine = M.q2_uniform(m, m, m)
+    L = G.Level(m, m, m, fine, None, None)
+    x = fine.reshape(-1, 3)
+    f = np.stack([1 + 2 * x[:, 0] - x[:, 1] * x[:, 2], x[:, 0] * x[:, 1], 3 * x[:, 2]], axis=1).reshape(-1)
+    coarse = G.inject_from_fine(L, f)
+    assert np.abs(G.prolong(m // 2, m // 2, m // 2, coarse) - f).max() < 1e-14
+
+
+def test_eta_coarsening_constant_and_mean():
+    m = 4
+    eta = np.full(27 * m ** 3, 7.0)
+    assert np.allclose(G.eta_coarsen(m, m, m, eta), 7.0)
+    eta = M.uniform(3, 27 * m ** 3, 1, 2)
+    ec = G.eta_coarsen(m, m, m, eta).reshape(-1, 27)
+    e = eta.reshape(m, m, m, 27)
+    # the centre point of coarse element 0 averages all 8 children
+    assert abs(ec[0, 13] - e[:2, :2, :2].mean()) < 1e-14
+    # corner point 0 of coarse element 0 sees only child (0,0,0)
+    assert abs(ec[0, 0] - e[0, 0, 0].mean()) < 1e-14
+
+
+def test_chebyshev_reduces_error_like_the_polynomial():
+    """On a level with exact eigen-bounds the Chebyshev smoother's error reduction obeys the
+    Chebyshev bound 1/T_k((emax+emin)/(emax-emin)) on the smoothed part of the spectrum."""
+    m = 2
+    P = M.config_jitter_iso(m)
+    L = G.Level(m, m, m, P.coords, None, P.eta)
+    n = P.coords.size
+    A = np.stack([L.matmult(np.eye(n)[j]) for j in range(n)], axis=1)
+    d = np.diag(A)
+    S = A / np.sqrt(d)[:, None] / np.sqrt(d)[None, :]
+    lam, V = np.linalg.eigh(0.5 * (S + S.T))
+    emax = lam[-1]
+    emin = 0.3 * emax
+    its = 4
+    # error of x = Cheb(b) against the exact solution, in the high part of the spectrum
+    xs = M.uniform(4, n, -1, 1)
+    b = A @ xs
+    x = G.chebyshev(L, b, None, 1.0 / d, emin, emax, its, True)
+    e0 = np.sqrt(d) * xs
+    e1 = np.sqrt(d) * (xs - x)
+    hi = lam >= emin
+    c0 = V.T @ e0
+    c1 = V.T @ e1
+    th = np.cosh((its + 1) * np.arccosh((emax + emin) / (emax - emin)))
+    ratio = np.linalg.norm(c1[hi]) / np.linalg.norm(c0[hi])
+    assert ratio <= 1.0 / th * 1.0001
+
+
+def test_vcycle_preconditions_gmres():
+    P = M.config_sinker(8, n_incl=2, contrast=1e2, bc="freeslip")
+    lv = G.hierarchy(8, 8, 8, P.coords, P.mask, P.eta, 3)
+    # eigen-bounds from a dense estimate on the two smoothed levels
+    emin, emax = [], []
+    for L in lv[:-1]:
+        n = L.n
+        rng = np.random.default_rng(0)
+        v = rng.standard_normal(n)
+        d = L.diag()
+        for _ in range(60):
+            w = L.matmult(v) / d
+            lam = np.linalg.norm(w) / np.linalg.norm(v)
+            v = w / np.linalg.norm(w)
+        emax.append(1.1 * lam)
+        emin.append(0.2 * lam)
+    V = G.VCycle(lv, emin, emax, 4)
+    xs = M.uniform(5, lv[0].n, -1, 1)
+    xs[P.mask > 0] = 0.0
+    b = lv[0].matmult(xs)
+    r0 = np.linalg.norm(b)
+    x = V(b)
+    assert np.linalg.norm(b - lv[0].matmult(x)) < 0.05 * r0
+    # as a GMRES preconditioner it reaches 1e-8 in a few tens of iterations
+    import scipy.sparse.linalg as sla
+
+    n = lv[0].n
+    A = sla.LinearOperator((n, n), matvec=lambda v: lv[0].matmult(v))
+    Mp = sla.LinearOperator((n, n), matvec=lambda v: V(v))
+    hist = []
+    x, info = sla.gmres(A, b, M=Mp, rtol=1e-8, restart=30, maxiter=100, callback=lambda r: hist.append(r),
+                        callback_type="pr_norm")
+    assert info == 0 and len(hist) <= 25
+    assert np.linalg.norm(b - lv[0].matmult(x)) <= 1e-8 * r0 * 1.01
