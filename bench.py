@@ -31,7 +31,7 @@ def algorithmic_bytes_a11(m, value_bytes=8):
 def build_problem(m):
     from ptq2 import mesh as M
 
-    # configs[1] recipe (affine+jitter mesh, 12 inclusions eta 1e3, log-uniform noise) at m^3
+    # affine+jitter mesh, 12 sphere inclusions with eta 1e3 and log-uniform noise, m^3 elements
     return M.config_sinker(m, n_incl=12, contrast=1e3, bc="none")
 
 
@@ -77,6 +77,52 @@ def cpu_baseline(seconds=12.0):
             "sample": "%d FP64 A11 applies on a 40^3-element Q2 mesh (%d velocity dofs), %.1f s" % (n, P.nvel, el)}
 
 
+def solve_comparison(m, nlevels, rtol=1e-8, restart=30, max_it=600):
+    """The viscous solve of the buoyancy problem (free-slip walls, 20 dense spheres with
+    viscosity contrast 1e4) with GMG-preconditioned FGMRES, once with FP64 and once with FP32
+    Chebyshev smoothers.  Wall time of each solve from x = 0 to rtol on the host clock
+    around synchronisations (the FGMRES control flow runs on the host)."""
+    import torch
+
+    import ptq2
+    from ptq2 import mesh as M
+    from ptq2.solver import ViscousSolver
+
+    t = time.perf_counter()
+    P, b = M.config_buoyancy(m)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(m, m, m, P.coords, P.mask)
+    ctx.set_viscosity(P.eta)
+    bd = torch.from_numpy(b).to("cuda:0")
+    del b
+    print("solve set-up %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
+    out = {"workload": "A11 viscous solve, %d^3 Q2 elements, free-slip walls, 20 spheres eta 1e4 / rho 1.1, "
+                       "buoyancy RHS, FGMRES(%d) + GMG V-cycle (%d levels, Chebyshev(4)-Jacobi), rtol %g"
+                       % (m, restart, nlevels, rtol)}
+    for fp32 in (False, True):
+        key = "fp32" if fp32 else "fp64"
+        t = time.perf_counter()
+        S = ViscousSolver(ctx, nlevels=nlevels, smooth_its=4, fp32=fp32)
+        setup = time.perf_counter() - t
+        S.vcycle(bd)  # warm the graph
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        x, info = S.solve(bd, rtol=rtol, max_it=max_it, restart=restart)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t
+        out[key + "_s"] = round(el, 4)
+        out[key + "_iterations"] = info["iterations"]
+        out[key + "_rel_residual"] = info["rel"]
+        out[key + "_setup_s"] = round(setup, 3)
+        print("%s smoother: %d its, rel %.3e, %.3f s" % (key, info["iterations"], info["rel"], el),
+              file=sys.stderr, flush=True)
+        S.destroy()
+        del x
+    out["speedup_fp32_vs_fp64"] = round(out["fp64_s"] / out["fp32_s"], 3)
+    ctx.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +131,11 @@ def main():
     ap.add_argument("--size", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp32", action="store_true", help="also time the FP32 smoother operator")
+    ap.add_argument("--solve", type=int, default=0, metavar="M",
+                    help="also time the FP32- vs FP64-smoothed GMG solve on an M^3 mesh (0 = skip)")
+    ap.add_argument("--solve-levels", type=int, default=6)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run only a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X; --gpus %d is not supported"
@@ -96,6 +147,15 @@ def main():
 
     import ptq2
     from ptq2 import _lib
+
+    if args.copy_probe:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": src.numel() * 8, "copy_probe_bytes_written": dst.numel() * 8}))
+        return
 
     m = args.size
     t_setup = time.perf_counter()
@@ -124,6 +184,11 @@ def main():
         extra["fp32_ms"] = round(ms32, 4)
         torch.cuda.synchronize()
         extra["fp32_rel_err_vs_fp64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
+
+    if args.solve:
+        del A, u, y
+        torch.cuda.empty_cache()
+        extra["solve"] = solve_comparison(args.solve, args.solve_levels)
 
     traffic = None
     prof = os.path.join(REPO, "profiles", "a11_fp64_128_traffic.json")

@@ -19,12 +19,22 @@ namespace ptq2 {
 constexpr int BE = 4;               // elements per brick edge
 constexpr int BEL = BE * BE * BE;   // elements per brick (= lanes per wave)
 constexpr int TN = 2 * BE + 1;      // tile nodes per edge
-constexpr int TROW = TN;            // node stride between tile rows
-constexpr int TPL = TN * TN;        // node stride between tile planes
-constexpr int TN3 = TN * TN * TN;   // nodes per tile
-constexpr int TCS = TN3 + 3;        // component stride in a tile (padded)
 constexpr int TROWS = TN * TN;      // rows per tile
-constexpr int RLEN = 3 * TN;        // doubles per row (interlaced nodes)
+constexpr int RLEN = 3 * TN;        // values per global row (interlaced nodes)
+
+// LDS tile layout, one array per component: value (x,y,z) at
+//   c*TCS + tx(x) + TRS*y + TPS*z,   tx(x) = x/2 for even x, 5 + x/2 for odd x.
+// Element (lx,ly,lz) of a lane reads its node (i,j,k) at lane offset
+//   lx + 2*TRS*ly + 2*TPS*lz  plus  xo(i) + TRS*j + TPS*k,  xo = {0,5,1}.
+// With TRS = 10, TPS = 92 and the lane -> lz order 0,2,1,3 (lane_elem), the 64
+// lane offsets of every 8-byte read fall in distinct banks within each group of 16
+// lanes (32 banks) and of 32 lanes (64 banks): the element loops are free of LDS
+// bank conflicts.  Two FP64 tiles take 39744 B, so four bricks fit a CU.
+constexpr int TRS = 10;             // row stride
+constexpr int TPS = 92;             // plane stride
+constexpr int TCS = TN * TPS;       // component stride
+__host__ __device__ constexpr int tx(int x) { return (x & 1) ? 5 + (x >> 1) : (x >> 1); }
+__host__ __device__ constexpr int xo(int i) { return i == 0 ? 0 : (i == 1 ? 5 : 1); }
 
 // Mesh box: NX x NY x NZ nodes, mx x my x mz owned elements, first node of
 // element (0,0,0) at (i0,j0,k0); bricks nbx x nby x nbz.
@@ -94,47 +104,160 @@ __device__ __forceinline__ Tile tile_of(const Box &b, const BrickId &br) {
   return t;
 }
 
-// Stage an interlaced 3-component nodal field into an LDS tile [comp][z][y][x].
-// Rows of the tile are contiguous runs of 27 values in global memory.  When mask
-// is given, Dirichlet dofs read as zero (BCListInsertLocalZero,
-// src/dmda_bcs.c:581-608).  Tile nodes outside the box read as zero.
-template <typename T, typename S>
-__device__ __forceinline__ void load_tile(const Box &b, const Tile &t, const S *__restrict__ g,
-                                          const uint8_t *__restrict__ mask, T *__restrict__ s, int lane) {
-  for (int f = lane; f < TROWS * RLEN; f += BEL) {
-    int r = f / RLEN, col = f - r * RLEN;
-    int jj = r % TN, kk = r / TN;
-    int ii = col / 3, c = col - 3 * ii;
-    T v = T(0);
-    if (ii < t.tnx && jj < t.tny && kk < t.tnz) {
-      int64_t gi = 3 * (t.node0 + ii + (int64_t)b.NX * (jj + (int64_t)b.NY * kk)) + c;
-      v = static_cast<T>(g[gi]);
-      if (mask && mask[gi]) v = T(0);
-    }
-    s[c * TCS + kk * TPL + jj * TROW + ii] = v;
+// Tile traffic.  A tile is 81 rows (y,z) of 27 values (3 components of 9 nodes), and
+// a row is a contiguous run of 27 values in global memory.  Lanes 0-53 walk the rows
+// in pairs: lane = 27 p + col takes column col of row 2 it + p at step it (41 steps).
+// Each lane's column terms are computed once; a step costs a few integer adds.  All
+// loads of a pass are issued before the first LDS store (one exposed memory latency
+// per tile instead of one per value).
+constexpr int TPAIRS = (TROWS + 1) / 2;
+struct TileLane {
+  int p;           // row parity of this lane (0/1), 2 = idle lane
+  int lds_col;     // LDS offset of the column: c*TCS + tx(ii)
+  bool col_ok;     // column inside the box
+  bool shx;        // node column shared with the x-neighbour brick
+  int64_t gcol;    // global index of the column in row (0,0)
+  int64_t sy, sz;  // global strides of a tile row in y and z
+};
+__device__ __forceinline__ TileLane tile_lane(const Box &b, const Tile &t, const BrickId &br, int lane) {
+  TileLane L;
+  L.p = lane < 2 * RLEN ? lane / RLEN : 2;
+  const int col = lane - RLEN * (lane < 2 * RLEN ? L.p : 0);
+  const int ii = col / 3, c = col - 3 * ii;
+  L.lds_col = c * TCS + tx(ii);
+  L.col_ok = L.p < 2 && ii < t.tnx;
+  L.shx = (ii == 0 && br.bx > 0) || (ii == 2 * BE && br.bx < b.nbx - 1);
+  // the global column is clamped into the box, so every load address is valid
+  const int colc = L.p < 2 ? 3 * min(ii, t.tnx - 1) + c : 0;
+  L.gcol = 3 * t.node0 + colc;
+  L.sy = 3 * (int64_t)b.NX;
+  L.sz = 3 * (int64_t)b.NX * (int64_t)b.NY;
+  return L;
+}
+// row (y,z) of step it for this lane
+__device__ __forceinline__ void tile_row(int it, const TileLane &L, int &y, int &z) {
+  y = (2 * it) % TN + L.p;
+  z = (2 * it) / TN;
+  if (y >= TN) {
+    y -= TN;
+    z += 1;
   }
+}
+// global index of row (y,z) clamped into the box (always a valid address)
+__device__ __forceinline__ int64_t tile_gidx(const Tile &t, const TileLane &L, int y, int z) {
+  return L.gcol + L.sy * min(y, t.tny - 1) + L.sz * min(z, t.tnz - 1);
+}
+
+// Stage an interlaced 3-component nodal field into an LDS tile.  When mask is given,
+// Dirichlet dofs read as zero (BCListInsertLocalZero, src/dmda_bcs.c:581-608).  Tile
+// nodes outside the box read as zero.
+template <bool HM, typename T, typename S>
+__device__ __forceinline__ void load_tile_m(const Box &b, const Tile &t, const BrickId &br, const S *__restrict__ g,
+                                            const uint8_t *__restrict__ mask, T *__restrict__ s, int lane) {
+  const TileLane L = tile_lane(b, t, br, lane);
+  // all loads first (addresses clamped into the box, so none is conditional), then
+  // the selects and LDS stores
+  S v[TPAIRS];
+  int mk[TPAIRS];  // full registers: packed byte loads would serialize
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    int y, z;
+    tile_row(it, L, y, z);
+    const int64_t gi = tile_gidx(t, L, y, z);
+    v[it] = g[gi];
+    mk[it] = HM ? (int)mask[gi] : 0;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    int y, z;
+    tile_row(it, L, y, z);
+    const bool ok = L.col_ok && y < t.tny && z < t.tnz;
+    if (L.p < 2 && z < TN) s[L.lds_col + TRS * y + TPS * z] = (ok && !mk[it]) ? static_cast<T>(v[it]) : T(0);
+  }
+}
+
+template <typename T, typename S>
+__device__ __forceinline__ void load_tile(const Box &b, const Tile &t, const BrickId &br, const S *__restrict__ g,
+                                          const uint8_t *__restrict__ mask, T *__restrict__ s, int lane) {
+  if (mask)
+    load_tile_m<true>(b, t, br, g, mask, s, lane);
+  else
+    load_tile_m<false>(b, t, br, g, mask, s, lane);
+}
+
+// Two fields over the same tile in one pass; mask2 applies to the second (if HM).
+template <bool HM, typename T, typename S>
+__device__ __forceinline__ void load_tile2_m(const Box &b, const Tile &t, const BrickId &br, const S *__restrict__ g1,
+                                             T *__restrict__ s1, const S *__restrict__ g2,
+                                             const uint8_t *__restrict__ mask2, T *__restrict__ s2, int lane) {
+  const TileLane L = tile_lane(b, t, br, lane);
+  S v1[TPAIRS], v2[TPAIRS];
+  int mk[TPAIRS];
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    int y, z;
+    tile_row(it, L, y, z);
+    const int64_t gi = tile_gidx(t, L, y, z);
+    v1[it] = g1[gi];
+    v2[it] = g2[gi];
+    mk[it] = HM ? (int)mask2[gi] : 0;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    int y, z;
+    tile_row(it, L, y, z);
+    const bool ok = L.col_ok && y < t.tny && z < t.tnz;
+    if (L.p < 2 && z < TN) {
+      const int o = L.lds_col + TRS * y + TPS * z;
+      s1[o] = ok ? static_cast<T>(v1[it]) : T(0);
+      s2[o] = (ok && !mk[it]) ? static_cast<T>(v2[it]) : T(0);
+    }
+  }
+}
+
+template <typename T, typename S>
+__device__ __forceinline__ void load_tile2(const Box &b, const Tile &t, const BrickId &br, const S *__restrict__ g1,
+                                           T *__restrict__ s1, const S *__restrict__ g2,
+                                           const uint8_t *__restrict__ mask2, T *__restrict__ s2, int lane) {
+  if (mask2)
+    load_tile2_m<true>(b, t, br, g1, s1, g2, mask2, s2, lane);
+  else
+    load_tile2_m<false>(b, t, br, g1, s1, g2, mask2, s2, lane);
 }
 
 // Write an LDS output tile back.  Bricks of one colour never share a node.  In
 // overwrite mode the first brick to touch a node (parity 0 along every axis on
 // which the node is shared, i.e. the lowest colour) stores, the later colours add.
+// The global values to add to are all loaded before the first store.
 template <typename T>
 __device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, const Tile &t, const T *__restrict__ s,
                                            T *__restrict__ g, bool add_mode, int lane) {
-  for (int f = lane; f < TROWS * RLEN; f += BEL) {
-    int r = f / RLEN, col = f - r * RLEN;
-    int jj = r % TN, kk = r / TN;
-    int ii = col / 3, c = col - 3 * ii;
-    if (ii < t.tnx && jj < t.tny && kk < t.tnz) {
-      int64_t gi = 3 * (t.node0 + ii + (int64_t)b.NX * (jj + (int64_t)b.NY * kk)) + c;
-      T v = s[c * TCS + kk * TPL + jj * TROW + ii];
-      bool shx = (ii == 0 && br.bx > 0) || (ii == 2 * BE && br.bx < b.nbx - 1);
-      bool shy = (jj == 0 && br.by > 0) || (jj == 2 * BE && br.by < b.nby - 1);
-      bool shz = (kk == 0 && br.bz > 0) || (kk == 2 * BE && br.bz < b.nbz - 1);
-      bool first = (!shx || !(br.bx & 1)) && (!shy || !(br.by & 1)) && (!shz || !(br.bz & 1));
-      if (add_mode || !first) v += g[gi];
-      g[gi] = v;
-    }
+  // opaque copy of the lane: keeps the compiler from sharing this tile's address
+  // arithmetic with the loads at the top of the kernel (which would keep it live
+  // across the whole element computation)
+  asm volatile("" : "+v"(lane));
+  const TileLane L = tile_lane(b, t, br, lane);
+  T v[TPAIRS];
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    int y, z;
+    tile_row(it, L, y, z);
+    const bool ok = L.col_ok && y < t.tny && z < t.tnz;
+    const bool shy = (y == 0 && br.by > 0) || (y == 2 * BE && br.by < b.nby - 1);
+    const bool shz = (z == 0 && br.bz > 0) || (z == 2 * BE && br.bz < b.nbz - 1);
+    const bool first = (!L.shx || !(br.bx & 1)) && (!shy || !(br.by & 1)) && (!shz || !(br.bz & 1));
+    const bool rd = ok && (add_mode || !first);
+    const T val = g[tile_gidx(t, L, y, z)];
+    v[it] = rd ? val : T(0);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    int y, z;
+    tile_row(it, L, y, z);
+    if (L.col_ok && y < t.tny && z < t.tnz) g[tile_gidx(t, L, y, z)] = v[it] + s[L.lds_col + TRS * y + TPS * z];
   }
 }
 
@@ -150,24 +273,35 @@ __device__ __forceinline__ void lds_add(float *p, float v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// lane -> element inside the brick and its LDS tile base offset
+// lane -> element (lx,ly,lz) inside the brick; lanes 16-31 take lz = 2 and
+// lanes 32-47 lz = 1 (bank-conflict-free tile reads, see the layout above).
+struct LaneElem {
+  int lx, ly, lz;
+};
+__host__ __device__ __forceinline__ LaneElem lane_elem(int lane) {
+  const int g = lane >> 4;
+  return LaneElem{lane & (BE - 1), (lane >> 2) & (BE - 1), ((g & 1) << 1) | (g >> 1)};
+}
 __device__ __forceinline__ int lane_base(int lane) {
-  int lx = lane & (BE - 1), ly = (lane >> 2) & (BE - 1), lz = lane >> 4;
-  return 2 * lz * TPL + 2 * ly * TROW + 2 * lx;
+  const LaneElem e = lane_elem(lane);
+  return e.lx + 2 * TRS * e.ly + 2 * TPS * e.lz;
 }
 __device__ __forceinline__ bool lane_active(const Tile &t, int lane) {
-  int lx = lane & (BE - 1), ly = (lane >> 2) & (BE - 1), lz = lane >> 4;
-  return lx < t.nex && ly < t.ney && lz < t.nez;
+  const LaneElem e = lane_elem(lane);
+  return e.lx < t.nex && e.ly < t.ney && e.lz < t.nez;
 }
 __device__ __forceinline__ int64_t lane_element(const Box &b, const Tile &t, int lane) {
-  int lx = lane & (BE - 1), ly = (lane >> 2) & (BE - 1), lz = lane >> 4;
-  return (int64_t)(t.ex0 + lx) + (int64_t)b.mx * ((int64_t)(t.ey0 + ly) + (int64_t)b.my * (int64_t)(t.ez0 + lz));
+  const LaneElem e = lane_elem(lane);
+  return (int64_t)(t.ex0 + e.lx) + (int64_t)b.mx * ((int64_t)(t.ey0 + e.ly) + (int64_t)b.my * (int64_t)(t.ez0 + e.lz));
 }
+
+// offset of element-local node (i,j,k), component c, from the lane base
+__host__ __device__ constexpr int toff(int c, int i, int j, int k) { return c * TCS + xo(i) + j * TRS + k * TPS; }
 
 // Tile value at element-local node (i,j,k), component c.
 template <typename T>
 __device__ __forceinline__ T tv(const T *s, int base, int c, int i, int j, int k) {
-  return s[c * TCS + base + k * TPL + j * TROW + i];
+  return s[base + toff(c, i, j, k)];
 }
 
 // Row data of one field for quadrature row (qb,qc): for each component c and
@@ -376,6 +510,50 @@ __device__ __forceinline__ void row_data_rt(const T *s, int base, const Coef1 &c
       r2[c][i] = fmac(cb.b[2], td[2], fmac(cb.b[1], td[1], static_cast<T>(cb.b[0]) * td[0]));
       __builtin_amdgcn_sched_barrier(0);
     }
+}
+
+// One tile column: the 9 values (j,k) of component c at node column i.
+template <typename T>
+__device__ __forceinline__ void ld_col(const T *s, int base, int g, T v[9]) {
+  const int c = g / 3, i = g % 3;
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) v[3 * k + j] = tv(s, base, c, i, j, k);
+}
+// contraction of one column with the row's z (cc) and y (cb) coefficients
+template <typename T>
+__device__ __forceinline__ void col_contract(const T v[9], const Coef1 &cb, const Coef1 &cc, T &r0, T &r1, T &r2) {
+  T tb[3], td[3];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    tb[j] = fmac(cc.b[2], v[6 + j], fmac(cc.b[1], v[3 + j], static_cast<T>(cc.b[0]) * v[j]));
+    td[j] = fmac(cc.d[2], v[6 + j], fmac(cc.d[1], v[3 + j], static_cast<T>(cc.d[0]) * v[j]));
+  }
+  r0 = fmac(cb.b[2], tb[2], fmac(cb.b[1], tb[1], static_cast<T>(cb.b[0]) * tb[0]));
+  r1 = fmac(cb.d[2], tb[2], fmac(cb.d[1], tb[1], static_cast<T>(cb.d[0]) * tb[0]));
+  r2 = fmac(cb.b[2], td[2], fmac(cb.b[1], td[1], static_cast<T>(cb.b[0]) * td[0]));
+}
+// Row data of one field with the tile reads software-pipelined over two column
+// buffers: on entry `first` holds column 0 of field s; column g is contracted from
+// `first` (g even) or `second` (g odd) while the next column is read into the other
+// buffer.  Column 8 issues the reads of column 0 of field s_next into `second`, so the
+// next phase passes the buffers swapped.
+template <typename T>
+__device__ __forceinline__ void row_data_pipe(const T *s, const T *s_next, int base, const Coef1 &cb,
+                                              const Coef1 &cc, T first[9], T second[9], T r0[3][3], T r1[3][3],
+                                              T r2[3][3]) {
+#pragma unroll
+  for (int g = 0; g < 9; g++) {
+    T *cur = (g & 1) ? second : first;
+    T *nxt = (g & 1) ? first : second;
+    if (g < 8)
+      ld_col(s, base, g + 1, nxt);
+    else
+      ld_col(s_next, base, 0, nxt);
+    col_contract(cur, cb, cc, r0[g / 3][g % 3], r1[g / 3][g % 3], r2[g / 3][g % 3]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
 }
 
 // y-transpose of a finished row (runtime qb) into the slab accumulators

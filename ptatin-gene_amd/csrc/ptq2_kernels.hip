@@ -23,20 +23,23 @@ __device__ __forceinline__ void geo_qp(const double xr0[3][3], const double xr1[
 
 // one quadrature row (runtime qb,qc) of the FP64 A11 element: geometry of its three
 // points from the coordinate tile, velocity gradients from the velocity tile, the
-// stress action and the x-transpose into P, then the y-transpose into the slab sums.
+// stress action and the x-transpose into P, then the y- and z-transposes into Y.
+// On entry ca holds column 0 of the coordinate tile; on exit it holds it again (for
+// the next row), its reads in flight behind the stress work.
 __device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, int base, const Coef1 &cb,
-                                          const Coef1 &cc, double wbc, const double e3[3], double Y[3][3][3][3]) {
+                                          const Coef1 &cc, double wbc, const double e3[3], double ca[9],
+                                          double cb9[9], double Y[3][3][3][3]) {
   double A0[3][3], A1[3][3], A2[3][3], s0, s1, s2;
   {
     double xr0[3][3], xr1[3][3], xr2[3][3];
-    row_data_rt(Xs, base, cb, cc, xr0, xr1, xr2);
+    row_data_pipe(Xs, Us, base, cb, cc, ca, cb9, xr0, xr1, xr2);
     geo_qp<0>(xr0, xr1, xr2, wbc, e3[0], A0, s0);
     geo_qp<1>(xr0, xr1, xr2, wbc, e3[1], A1, s1);
     geo_qp<2>(xr0, xr1, xr2, wbc, e3[2], A2, s2);
   }
   row_fence();
   double ur0[3][3], ur1[3][3], ur2[3][3];
-  row_data_rt(Us, base, cb, cc, ur0, ur1, ur2);
+  row_data_pipe(Us, Xs, base, cb, cc, cb9, ca, ur0, ur1, ur2);
   double P0[3][3], P1[3][3], P2[3][3];
   zero33(P0); zero33(P1); zero33(P2);
   double g[3][3], dv[3][3];
@@ -48,6 +51,7 @@ __device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, in
   row_fence();
 }
 
+// add the element's 81 outputs into the LDS output tile
 template <typename T>
 __device__ __forceinline__ void emit_element(T *Ys, int base, const T Y[3][3][3][3]) {
 #pragma unroll
@@ -57,7 +61,7 @@ __device__ __forceinline__ void emit_element(T *Ys, int base, const T Y[3][3][3]
 #pragma unroll
       for (int j = 0; j < 3; j++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) lds_add(&Ys[c * TCS + base + k * TPL + j * TROW + i], Y[c][k][j][i]);
+        for (int i = 0; i < 3; i++) lds_add(&Ys[base + toff(c, i, j, k)], Y[c][k][j][i]);
 }
 
 __device__ __forceinline__ void load9(const double *__restrict__ ep, int qc, double *e9) {
@@ -82,8 +86,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
   const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
   double en[3];
   load3(ep, 0, en);
-  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
-  load_tile(b, t, u, mask, Us, lane);
+  load_tile2(b, t, br, coords, Xs, u, mask, Us, lane);
   __syncthreads();
   const int base = lane_base(lane);
   const bool active = lane_active(t, lane);
@@ -97,12 +100,14 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
   if (active) {
+    double ca[9], cb9[9];
+    ld_col(Xs, base, 0, ca);
 #pragma nounroll
     for (int row = 0; row < 9; row++) {
       const int qb = row % 3, qc = row / 3;
       double e3[3] = {en[0], en[1], en[2]};
       if (row < 8) load3(ep, row + 1, en);
-      a11_row64(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, Y);
+      a11_row64(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y);
     }
   }
   __syncthreads();
@@ -119,20 +124,17 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
 // [brick][q][m][lane] so that every load instruction is one contiguous 256-byte
 // segment.  Then dv = F^T (M + M^T) with M = F g (diagonal doubled), all FP32.
 
+// one quadrature point of the FP32 A11 row: F = sqrt(w eta / det) adj(J) (9 floats)
 template <int qa>
 __device__ __forceinline__ void a11_qp32(const float ur0[3][3], const float ur1[3][3], const float ur2[3][3],
-                                         const float *__restrict__ fp, float P0[3][3], float P1[3][3],
-                                         float P2[3][3]) {
-  float F[3][3];
-#pragma unroll
-  for (int m = 0; m < 9; m++) F[m / 3][m % 3] = fp[(qa * 9 + m) * BEL];
+                                         const float F9[9], float P0[3][3], float P1[3][3], float P2[3][3]) {
   float g[3][3], dv[3][3];
   qp_grad<qa>(ur0, ur1, ur2, g);
   float M[3][3];
 #pragma unroll
   for (int k = 0; k < 3; k++)
 #pragma unroll
-    for (int l = 0; l < 3; l++) M[k][l] = g[0][l] * F[k][0] + g[1][l] * F[k][1] + g[2][l] * F[k][2];
+    for (int l = 0; l < 3; l++) M[k][l] = g[0][l] * F9[3 * k] + g[1][l] * F9[3 * k + 1] + g[2][l] * F9[3 * k + 2];
   float S[3][3];
   S[0][0] = 2.0f * M[0][0];
   S[1][1] = 2.0f * M[1][1];
@@ -143,8 +145,14 @@ __device__ __forceinline__ void a11_qp32(const float ur0[3][3], const float ur1[
 #pragma unroll
   for (int k = 0; k < 3; k++)
 #pragma unroll
-    for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * F[0][k] + S[1][l] * F[1][k] + S[2][l] * F[2][k];
+    for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * F9[k] + S[1][l] * F9[3 + k] + S[2][l] * F9[6 + k];
   acc_row<qa>(dv, P0, P1, P2);
+}
+
+// the 27 factors of quadrature row `row` (brick-major, lane-minor: coalesced)
+__device__ __forceinline__ void load_fac_row(const float *__restrict__ fbase, int row, float F[27]) {
+#pragma unroll
+  for (int n = 0; n < 27; n++) F[n] = fbase[(int64_t)(27 * row + n) * BEL];
 }
 
 __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
@@ -154,11 +162,13 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   const int lane = threadIdx.x;
   const BrickId br = brick_of(b, color, blockIdx.x);
   const Tile t = tile_of(b, br);
-  load_tile(b, t, u, mask, Us, lane);
+  const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
+  float Fn[27];
+  load_fac_row(fbase, 0, Fn);
+  load_tile(b, t, br, u, mask, Us, lane);
   __syncthreads();
   const int base = lane_base(lane);
   const bool active = lane_active(t, lane);
-  const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
   float Y[3][3][3][3];
 #pragma unroll
   for (int c = 0; c < 3; c++)
@@ -169,30 +179,29 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
   if (active) {
+    float ca[9], cb9[9];
+    ld_col(Us, base, 0, ca);
 #pragma nounroll
-    for (int qc = 0; qc < 3; qc++) {
-      const Coef1 cc = coef1(qc);
-      float RB[3][3][3], RD[3][3][3];
+    for (int row = 0; row < 9; row++) {
+      const int qb = row % 3, qc = row / 3;
+      const Coef1 cb = coef1(qb), cc = coef1(qc);
+      float F[27];
 #pragma unroll
-      for (int c = 0; c < 3; c++)
+      for (int n = 0; n < 27; n++) F[n] = Fn[n];
+      if (row < 8) load_fac_row(fbase, row + 1, Fn);
+      float ur0[3][3], ur1[3][3], ur2[3][3];
+      // column 0 for the next row lands in cb9; swap roles by copying back (9 moves)
+      row_data_pipe(Us, Us, base, cb, cc, ca, cb9, ur0, ur1, ur2);
 #pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0f;
-#pragma nounroll
-      for (int qb = 0; qb < 3; qb++) {
-        const Coef1 cb = coef1(qb);
-        const float *fp = fbase + (int64_t)(3 * (qb + 3 * qc)) * 9 * BEL;
-        float ur0[3][3], ur1[3][3], ur2[3][3];
-        row_data_rt(Us, base, cb, cc, ur0, ur1, ur2);
-        float P0[3][3], P1[3][3], P2[3][3];
-        zero33(P0); zero33(P1); zero33(P2);
-        a11_qp32<0>(ur0, ur1, ur2, fp, P0, P1, P2);
-        a11_qp32<1>(ur0, ur1, ur2, fp, P0, P1, P2);
-        a11_qp32<2>(ur0, ur1, ur2, fp, P0, P1, P2);
-        acc_slab_rt(cb, P0, P1, P2, RB, RD);
-      }
-      acc_elem_rt(cc, RB, RD, Y);
+      for (int n = 0; n < 9; n++) ca[n] = cb9[n];
+      float P0[3][3], P1[3][3], P2[3][3];
+      zero33(P0); zero33(P1); zero33(P2);
+      a11_qp32<0>(ur0, ur1, ur2, F, P0, P1, P2);
+      a11_qp32<1>(ur0, ur1, ur2, F + 9, P0, P1, P2);
+      a11_qp32<2>(ur0, ur1, ur2, F + 18, P0, P1, P2);
+      row_fence();
+      acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
+      row_fence();
     }
   }
   __syncthreads();
@@ -237,7 +246,7 @@ __global__ __launch_bounds__(BEL) void k_build_fac32(Box b, const double *__rest
   const int lane = threadIdx.x;
   const BrickId br = brick_lin(b, blockIdx.x);
   const Tile t = tile_of(b, br);
-  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  load_tile(b, t, br, coords, (const uint8_t *)nullptr, Xs, lane);
   __syncthreads();
   if (!lane_active(t, lane)) return;
   const int base = lane_base(lane);
@@ -294,7 +303,7 @@ __global__ __launch_bounds__(BEL) void k_check_mesh(Box b, const double *__restr
   const int lane = threadIdx.x;
   const BrickId br = brick_lin(b, blockIdx.x);
   const Tile t = tile_of(b, br);
-  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  load_tile(b, t, br, coords, (const uint8_t *)nullptr, Xs, lane);
   __syncthreads();
   if (!lane_active(t, lane)) return;
   const int base = lane_base(lane);
@@ -372,8 +381,7 @@ __global__ __launch_bounds__(BEL) void k_a21(Box b, const double *__restrict__ c
   const int lane = threadIdx.x;
   const BrickId br = brick_lin(b, blockIdx.x);
   const Tile t = tile_of(b, br);
-  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
-  load_tile(b, t, u, mask, Us, lane);
+  load_tile2(b, t, br, coords, Xs, u, mask, Us, lane);
   __syncthreads();
   if (!lane_active(t, lane)) return;
   const int base = lane_base(lane);
@@ -444,7 +452,7 @@ __global__ __launch_bounds__(BEL) void k_a12(Box b, int color, const double *__r
   const int lane = threadIdx.x;
   const BrickId br = brick_of(b, color, blockIdx.x);
   const Tile t = tile_of(b, br);
-  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  load_tile(b, t, br, coords, (const uint8_t *)nullptr, Xs, lane);
   __syncthreads();
   const int base = lane_base(lane);
   const bool active = lane_active(t, lane);
@@ -525,7 +533,7 @@ __global__ __launch_bounds__(BEL) void k_diag(Box b, int color, const double *__
   const int lane = threadIdx.x;
   const BrickId br = brick_of(b, color, blockIdx.x);
   const Tile t = tile_of(b, br);
-  load_tile(b, t, coords, (const uint8_t *)nullptr, Xs, lane);
+  load_tile(b, t, br, coords, (const uint8_t *)nullptr, Xs, lane);
   __syncthreads();
   const int base = lane_base(lane);
   const bool active = lane_active(t, lane);

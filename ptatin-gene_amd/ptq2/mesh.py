@@ -133,6 +133,64 @@ def qp_coords(mx, my, mz, coords):
     return x3.reshape(mx * my * mz, 27, 3)
 
 
+def _lattice_to_qp(mx, my, mz, g, ops):
+    """Apply 1D operators (3x3: quadrature point x element node) along x, y, z of a nodal
+    field on the Q2 lattice g (nz, ny, nx, k); result (mx*my*mz, 27, k), q = a + 3b + 9c."""
+    Ox, Oy, Oz = ops
+    x1 = np.stack([Ox[q, 0] * g[:, :, 0:-1:2] + Ox[q, 1] * g[:, :, 1::2] + Ox[q, 2] * g[:, :, 2::2]
+                   for q in range(3)], axis=3)  # (nz, ny, mx, qa, k)
+    x2 = np.stack([Oy[q, 0] * x1[:, 0:-1:2] + Oy[q, 1] * x1[:, 1::2] + Oy[q, 2] * x1[:, 2::2]
+                   for q in range(3)], axis=3)  # (nz, my, mx, qb, qa, k)
+    del x1
+    x3 = np.stack([Oz[q, 0] * x2[0:-1:2] + Oz[q, 1] * x2[1::2] + Oz[q, 2] * x2[2::2]
+                   for q in range(3)], axis=3)  # (mz, my, mx, qc, qb, qa, k)
+    return x3.reshape(mx * my * mz, 27, -1)
+
+
+def qp_jacobian_det(mx, my, mz, coords):
+    """det(dx/dxi) at the 27 quadrature points of every element, shape (E, 27)."""
+    B, D = basis_1d()
+    g = np.asarray(coords, dtype=np.float64).reshape(2 * mz + 1, 2 * my + 1, 2 * mx + 1, 3)
+    jx = _lattice_to_qp(mx, my, mz, g, (D, B, B))
+    jy = _lattice_to_qp(mx, my, mz, g, (B, D, B))
+    jz = _lattice_to_qp(mx, my, mz, g, (B, B, D))
+    return np.einsum("eqi,eqi->eq", jx, np.cross(jy, jz))
+
+
+def body_force(mx, my, mz, coords, fu):
+    """Consistent nodal body force b[3n+d] = sum_q w_q detJ_q N_n(x_q) fu[e,q,d] of every
+    element, summed over elements: the Be term of FormFunctionLocal_U
+    (src/stokes_form_function.c:219-224).  fu has shape (E, 27, 3)."""
+    B, _ = basis_1d()
+    W = (GAUSS_W[None, None, :] * GAUSS_W[None, :, None] * GAUSS_W[:, None, None]).reshape(27)
+    f = (W[None, :] * qp_jacobian_det(mx, my, mz, coords))[:, :, None] * np.asarray(fu).reshape(-1, 27, 3)
+    f = f.reshape(mz, my, mx, 3, 3, 3, 3)  # (ez, ey, ex, qc, qb, qa, d)
+    nx, ny, nz = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    # transpose of the interpolation, one direction at a time, scattering into the lattice
+    t = np.einsum("zyxcbad,ai->zyxcbid", f, B)
+    del f
+    ox = np.zeros((mz, my, 3, 3, nx, 3))
+    tt = np.moveaxis(t, 2, 5)  # (ez, ey, qc, qb, i, ex, d)
+    del t
+    for i in range(3):
+        ox[:, :, :, :, i:i + 2 * mx:2, :] += tt[:, :, :, :, i]
+    del tt
+    t = np.einsum("zycbxd,bj->zycjxd", ox, B)  # (ez, ey, qc, j, nx, d)
+    del ox
+    oy = np.zeros((mz, 3, ny, nx, 3))
+    tt = np.moveaxis(t, 1, 3)  # (ez, qc, j, ey, nx, d)
+    del t
+    for j in range(3):
+        oy[:, :, j:j + 2 * my:2, :, :] += tt[:, :, j]
+    del tt
+    t = np.einsum("zcyxd,ck->zkyxd", oy, B)  # (ez, k, ny, nx, d)
+    del oy
+    out = np.zeros((nz, ny, nx, 3))
+    for k in range(3):
+        out[k:k + 2 * mz:2] += t[:, k]
+    return out.reshape(-1)
+
+
 def sphere_inclusions(n, rmin, rmax, seed, lo=0.0, hi=1.0):
     """n spheres, radius U[rmin,rmax], centre U[lo+r, hi-r]^3 (SplitMix64 stream `seed`)."""
     u = uniform(seed, 4 * n).reshape(n, 4)
@@ -217,3 +275,65 @@ def config_sinker(m, n_incl=12, contrast=1e3, rmin=0.05, rmax=0.15, seed_jitter=
     c = affine(c, SHEAR, [0.0, 0.0, 0.0])
     mask = None if bc == "none" else bc_mask(m, m, m, bc)
     return Problem(m, m, m, c, mask, eta)
+
+
+def _inclusion_field(xq, mx, my, mz, ez0, mzg, centres, radii, v_in, v_out):
+    """v_in at quadrature points inside any sphere, v_out elsewhere, for the mz element
+    layers starting at layer ez0 of a (jittered) unit-cube mesh of mx x my x mzg elements.
+    Each sphere only tests the elements whose lattice cell range can reach it (node
+    jitter below half a cell)."""
+    out = np.full((mz, my, mx, 27), float(v_out))
+    x = xq.reshape(mz, my, mx, 27, 3)
+    for cen, r in zip(centres, radii):
+        lo = np.floor((cen - r) * np.array([mx, my, mzg])).astype(int) - 1
+        hi = np.ceil((cen + r) * np.array([mx, my, mzg])).astype(int) + 1
+        x0, x1 = max(lo[0], 0), min(hi[0], mx)
+        y0, y1 = max(lo[1], 0), min(hi[1], my)
+        z0, z1 = max(lo[2] - ez0, 0), min(hi[2] - ez0, mz)
+        if x0 >= x1 or y0 >= y1 or z0 >= z1:
+            continue
+        sub = x[z0:z1, y0:y1, x0:x1]
+        inside = ((sub - cen) ** 2).sum(axis=-1) <= r * r
+        out[z0:z1, y0:y1, x0:x1][inside] = float(v_in)
+    return out.reshape(-1, 27)
+
+
+def config_buoyancy(m, n_spheres=20, contrast=1e4, drho=0.1, rmin=0.05, rmax=0.15, seed_jitter=1,
+                    seed_spheres=5, frac=0.1, bc="freeslip", threads=None):
+    """Viscous solve set-up: unit cube with jittered interior nodes (walls stay planar for the
+    free-slip conditions), `n_spheres` seeded spheres of viscosity `contrast` and density
+    1 + drho in a matrix of viscosity 1 and density 1, gravity (0,-1,0).  Returns the Problem
+    and the consistent buoyancy right-hand side (zero on Dirichlet dofs).  The element
+    z-slabs are processed by a thread pool (numpy releases the GIL)."""
+    import concurrent.futures
+    import os
+
+    c = jitter(m, m, m, q2_uniform(m, m, m), frac, 1.0 / (2 * m), seed_jitter)
+    cen, r = sphere_inclusions(n_spheres, rmin, rmax, seed_spheres)
+    n1 = 2 * m + 1
+    g = c.reshape(n1, n1, n1, 3)
+    eta = np.empty((m, m * m, 27))
+    slab = max(1, min(8, m // 4))
+    chunks = [(z0, min(z0 + slab, m)) for z0 in range(0, m, slab)]
+
+    def work(ch):
+        z0, z1 = ch
+        mzc = z1 - z0
+        cs = np.ascontiguousarray(g[2 * z0:2 * z1 + 1]).reshape(-1)
+        xq = qp_coords(m, m, mzc, cs)
+        eta[z0:z1] = _inclusion_field(xq, m, m, mzc, z0, m, cen, r, contrast, 1.0).reshape(mzc, m * m, 27)
+        rho = _inclusion_field(xq, m, m, mzc, z0, m, cen, r, 1.0 + drho, 1.0)
+        del xq
+        fu = np.zeros(rho.shape + (3,))
+        fu[:, :, 1] = -rho
+        return body_force(m, m, mzc, cs, fu).reshape(2 * mzc + 1, n1, n1, 3)
+
+    nt = threads or min(16, os.cpu_count() or 1)
+    b = np.zeros((n1, n1, n1, 3))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=nt) as ex:
+        for (z0, z1), part in zip(chunks, ex.map(work, chunks)):
+            b[2 * z0:2 * z1 + 1] += part
+    b = b.reshape(-1)
+    mask = bc_mask(m, m, m, bc)
+    b[mask > 0] = 0.0
+    return Problem(m, m, m, c, mask, eta.reshape(-1)), b

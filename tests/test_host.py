@@ -98,3 +98,73 @@ def test_qp_coords_reproduce_affine_map():
     assert np.abs(xq.reshape(-1) - ref).max() < 1e-14
     g = M.GAUSS_X
     assert np.allclose(M.qp_coords(m, m, m, M.q2_uniform(m, m, m))[0, 0], (1 + g[0]) / (2 * m))
+
+
+def _body_force_brute(m, coords, fu):
+    """Element-by-element, point-by-point evaluation of sum_q w_q detJ_q N_n(x_q) fu_q."""
+    B, D = M.basis_1d()
+    n1 = 2 * m + 1
+    g = coords.reshape(n1, n1, n1, 3)
+    b = np.zeros((n1, n1, n1, 3))
+    for ez in range(m):
+        for ey in range(m):
+            for ex in range(m):
+                e = ex + m * (ey + m * ez)
+                X = g[2 * ez:2 * ez + 3, 2 * ey:2 * ey + 3, 2 * ex:2 * ex + 3]  # (k, j, i, 3)
+                for c in range(3):
+                    for bq in range(3):
+                        for a in range(3):
+                            N = np.einsum("k,j,i->kji", B[c], B[bq], B[a])
+                            J = np.stack([np.einsum("k,j,i,kjid->d", B[c], B[bq], D[a], X),
+                                          np.einsum("k,j,i,kjid->d", B[c], D[bq], B[a], X),
+                                          np.einsum("k,j,i,kjid->d", D[c], B[bq], B[a], X)])
+                            w = M.GAUSS_W[a] * M.GAUSS_W[bq] * M.GAUSS_W[c]
+                            q = a + 3 * bq + 9 * c
+                            b[2 * ez:2 * ez + 3, 2 * ey:2 * ey + 3, 2 * ex:2 * ex + 3] += \
+                                (w * np.linalg.det(J)) * N[..., None] * fu[e, q][None, None, None, :]
+    return b.reshape(-1)
+
+
+def test_body_force_matches_pointwise_assembly():
+    m = 2
+    c = M.jitter(m, m, m, M.q2_uniform(m, m, m), 0.1, 1.0 / (2 * m), 7)
+    c = M.affine(c, M.SHEAR, [0.1, 0.0, -0.2])
+    fu = M.uniform(8, m ** 3 * 27 * 3, -1, 1).reshape(-1, 27, 3)
+    b = M.body_force(m, m, m, c, fu)
+    ref = _body_force_brute(m, c, fu)
+    assert np.abs(b - ref).max() <= 1e-14 * np.abs(ref).max()
+
+
+def test_body_force_integrates_linear_functions():
+    # sum_n b_n x_n = int fu . x dV exactly, since the Q2 basis reproduces x
+    m = 3
+    c = M.jitter(m, m, m, M.q2_uniform(m, m, m), 0.1, 1.0 / (2 * m), 9)
+    E = m ** 3
+    fu = M.uniform(10, E * 27 * 3, 0.5, 1.5).reshape(E, 27, 3)
+    b = M.body_force(m, m, m, c, fu).reshape(-1, 3)
+    W = (M.GAUSS_W[None, None, :] * M.GAUSS_W[None, :, None] * M.GAUSS_W[:, None, None]).reshape(27)
+    wdet = W[None, :] * M.qp_jacobian_det(m, m, m, c)
+    xq = M.qp_coords(m, m, m, c)
+    X = c.reshape(-1, 3)
+    for d in range(3):
+        lhs = (b[:, d] * X[:, 0]).sum()
+        rhs = (wdet * fu[:, :, d] * xq[:, :, 0]).sum()
+        assert lhs == pytest.approx(rhs, rel=1e-13)
+    assert wdet.sum() == pytest.approx(1.0, rel=1e-13)  # jitter keeps the boundary: volume 1
+
+
+@pytest.mark.parametrize("m", [6, 10])
+def test_buoyancy_config_chunked_matches_direct(m):
+    P, b = M.config_buoyancy(m, threads=3)
+    c = M.jitter(m, m, m, M.q2_uniform(m, m, m), 0.1, 1.0 / (2 * m), 1)
+    cen, r = M.sphere_inclusions(20, 0.05, 0.15, 5)
+    xq = M.qp_coords(m, m, m, c)
+    eta = M.viscosity_inclusions(xq, cen, r, 1e4, 1.0, None)
+    rho = M.viscosity_inclusions(xq, cen, r, 1.1, 1.0, None).reshape(-1, 27)
+    fu = np.zeros(rho.shape + (3,))
+    fu[:, :, 1] = -rho
+    ref = M.body_force(m, m, m, c, fu)
+    ref[P.mask > 0] = 0.0
+    assert np.array_equal(P.eta, eta)
+    assert np.abs(b - ref).max() <= 1e-15 * np.abs(ref).max()
+    assert np.all(b[P.mask > 0] == 0.0)
