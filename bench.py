@@ -123,6 +123,51 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=30, max_it=600):
     return out
 
 
+def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=600, restart=30):
+    """Independent buoyancy solves of uneven sizes (viscosity contrasts 1e2..1e6, seeded
+    sphere placement, FP32 smoothing), run one after another and then concurrently (one
+    stream and host thread per problem).  Host wall time of each variant."""
+    import torch
+
+    import ptq2
+    from ptq2 import mesh as M
+    from ptq2.solver import ViscousSolver, auto_levels, solve_batch
+
+    contrasts = [10.0 ** (2 + 4 * k / max(1, len(sizes) - 1)) for k in range(len(sizes))]
+    solvers, rhs, ctxs = [], [], []
+    t = time.perf_counter()
+    for k, (m, c) in enumerate(zip(sizes, contrasts)):
+        P, b = M.config_buoyancy(m, contrast=c, seed_spheres=100 + k)
+        ctx = ptq2.Context(0)
+        ctx.setup_mesh(m, m, m, P.coords, P.mask)
+        ctx.set_viscosity(P.eta)
+        ctxs.append(ctx)
+        solvers.append(ViscousSolver(ctx, nlevels=auto_levels(m), smooth_its=4, fp32=True))
+        rhs.append(torch.from_numpy(b).to("cuda:0"))
+    print("batch set-up %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
+    for S, b in zip(solvers, rhs):
+        S.vcycle(b)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    seq = [S.solve(b, rtol=rtol, max_it=max_it, restart=restart)[1] for S, b in zip(solvers, rhs)]
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t
+    t = time.perf_counter()
+    bat = solve_batch(solvers, rhs, rtol=rtol, max_it=max_it, restart=restart)
+    torch.cuda.synchronize()
+    t_bat = time.perf_counter() - t
+    out = {"sizes": list(sizes), "contrasts": [float("%.3g" % c) for c in contrasts],
+           "iterations": [i["iterations"] for i in seq], "rel_residual": [i["rel"] for i in seq],
+           "batched_iterations_equal": [a["iterations"] == b[1]["iterations"] for a, b in zip(seq, bat)],
+           "sequential_s": round(t_seq, 4), "concurrent_s": round(t_bat, 4),
+           "speedup": round(t_seq / t_bat, 3)}
+    for S in solvers:
+        S.destroy()
+    for c in ctxs:
+        c.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,7 +178,8 @@ def main():
     ap.add_argument("--fp32", action="store_true", help="also time the FP32 smoother operator")
     ap.add_argument("--solve", type=int, default=0, metavar="M",
                     help="also time the FP32- vs FP64-smoothed GMG solve on an M^3 mesh (0 = skip)")
-    ap.add_argument("--solve-levels", type=int, default=6)
+    ap.add_argument("--solve-levels", type=int, default=5)
+    ap.add_argument("--batch", action="store_true", help="also time 8 uneven solves, sequential vs concurrent")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -189,6 +235,8 @@ def main():
         del A, u, y
         torch.cuda.empty_cache()
         extra["solve"] = solve_comparison(args.solve, args.solve_levels)
+    if args.batch:
+        extra["batch"] = batch_comparison()
 
     traffic = None
     prof = os.path.join(REPO, "profiles", "a11_fp64_128_traffic.json")

@@ -100,9 +100,10 @@ int ptq2_a12_apply(ptq2_ctx *ctx, const double *p, double *yu, unsigned flags);
    (src/stokes_operators.c:1444-1509): Dirichlet columns of u zeroed. */
 int ptq2_a21_apply(ptq2_ctx *ctx, const double *u, double *yp, unsigned flags);
 /* full block [A11 A12; A21 0]: MFStokesWrapper_A_AVX (src/stokes_operators_avx.c:439-636);
-   PTQ2_MATMULT is MatMult_MFStokes_A (src/stokes_operators.c:893-979). */
-int ptq2_stokes_apply(ptq2_ctx *ctx, const double *u, const double *p, double *yu,
-                      double *yp, unsigned flags);
+   PTQ2_MATMULT is MatMult_MFStokes_A (src/stokes_operators.c:893-979).  With PTQ2_FP32
+   all four vectors are float and the block runs in one fused pass on FP32-stored
+   geometric factors (FP64 Jacobians rounded once per viscosity state); otherwise double. */
+int ptq2_stokes_apply(ptq2_ctx *ctx, const void *u, const void *p, void *yu, void *yp, unsigned flags);
 
 /* introspection (sizes, for callers that allocate device vectors) */
 int ptq2_sizes(const ptq2_ctx *ctx, int64_t *n_velocity_dofs, int64_t *n_pressure_dofs,

@@ -38,7 +38,7 @@ typedef struct {
 } ptq2_mg_opts;
 
 /* fill opts with the defaults: Chebyshev(4), FP32 smoothing, (0.2, 1.1) transform,
-   20 power iterations, graph replay, coarse dense limit 6000 dofs */
+   20 power iterations, graph replay, coarse dense limit 20000 dofs */
 void ptq2_mg_default_opts(ptq2_mg_opts *opts);
 
 /* Build the hierarchy under a set-up context (mesh with BC mask + viscosity): coarse

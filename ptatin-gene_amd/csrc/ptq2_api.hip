@@ -88,22 +88,27 @@ int clear_err(ptq2_ctx *c) {
 
 }  // namespace
 
-int ptq2_internal_build_fac32(ptq2_ctx *c);
-int ptq2_internal_build_fac32(ptq2_ctx *c) {
-  if (c->fac32_valid) return PTQ2_OK;
+int ptq2_internal_build_fac32(ptq2_ctx *c, bool block) {
+  if (c->fac32_valid && (!block || c->facb32_valid)) return PTQ2_OK;
   if (!c->fac32) {
     int rc = dev_alloc(c, (void **)&c->fac32, sizeof(float) * (size_t)c->nbricks * 243 * ptq2::BEL);
     if (rc) return rc;
   }
+  if (block && !c->facb32) {
+    int rc = dev_alloc(c, (void **)&c->facb32, sizeof(float) * (size_t)c->nbricks * 108 * ptq2::BEL);
+    if (rc) return rc;
+  }
   int rc = clear_err(c);
   if (rc) return rc;
-  HIPCHK(c, ptq2::launch_build_fac32(c->box, c->coords, c->eta_bm, c->fac32, c->err, c->stream));
+  HIPCHK(c, ptq2::launch_build_fac32(c->box, c->coords, c->eta_bm, c->fac32, block ? c->facb32 : nullptr, c->err,
+                                     c->stream));
   int bits = 0;
   rc = read_err(c, &bits);
   if (rc) return rc;
   if (bits & ptq2::ERRBIT_MESH) return fail(c, PTQ2_ERR_MESH, "non-positive Jacobian determinant");
   if (bits & ptq2::ERRBIT_VISCOSITY) return fail(c, PTQ2_ERR_VISCOSITY, "eta <= 0 or non-finite");
   c->fac32_valid = true;
+  if (block) c->facb32_valid = true;
   return PTQ2_OK;
 }
 
@@ -184,7 +189,7 @@ int ptq2_destroy(ptq2_ctx **pc) {
   ptq2_ctx *c = *pc;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm); dev_free(c->fac32);
+  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm); dev_free(c->fac32); dev_free(c->facb32);
   dev_free(c->err); dev_free(c->su); dev_free(c->sy); dev_free(c->sp); dev_free(c->syp);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -237,9 +242,9 @@ int ptq2_setup_mesh_box(ptq2_ctx *c, int mx, int my, int mz, int NX, int NY, int
 
   // release a previous mesh
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm); dev_free(c->fac32);
-  c->coords = nullptr; c->mask = nullptr; c->dir_idx = nullptr; c->eta_bm = nullptr; c->fac32 = nullptr;
-  c->mesh_ready = c->eta_ready = c->fac32_valid = c->has_state = false;
+  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm); dev_free(c->fac32); dev_free(c->facb32);
+  c->coords = nullptr; c->mask = nullptr; c->dir_idx = nullptr; c->eta_bm = nullptr; c->fac32 = nullptr; c->facb32 = nullptr;
+  c->mesh_ready = c->eta_ready = c->fac32_valid = c->facb32_valid = c->has_state = false;
   c->ndir = 0;
 
   Box b;
@@ -325,6 +330,7 @@ int ptq2_set_viscosity(ptq2_ctx *c, const double *eta, uint64_t state, unsigned 
   if (rc) return rc;
   if (e != hipSuccess) return fail(c, PTQ2_ERR_DEVICE, "eta permutation: %s", hipGetErrorString(e));
   c->fac32_valid = false;
+  c->facb32_valid = false;
   if (bits & ptq2::ERRBIT_VISCOSITY) {
     c->eta_ready = false;
     return fail(c, PTQ2_ERR_VISCOSITY, "eta <= 0 or non-finite at a quadrature point");
@@ -456,37 +462,53 @@ int ptq2_a21_apply(ptq2_ctx *c, const double *u, double *yp, unsigned flags) {
   return PTQ2_OK;
 }
 
-int ptq2_stokes_apply(ptq2_ctx *c, const double *u, const double *p, double *yu, double *yp, unsigned flags) {
+int ptq2_stokes_apply(ptq2_ctx *c, const void *u, const void *p, void *yu, void *yp, unsigned flags) {
   int rc = check_ready(c, true);
   if (rc) return rc;
   if (!u || !p || !yu || !yp) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
-  if (flags & ~(PTQ2_DEVICE | PTQ2_MATMULT | PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
+  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
   if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
   if ((rc = set_device(c))) return rc;
-  const bool dev = flags & PTQ2_DEVICE, mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
+  const bool dev = flags & PTQ2_DEVICE, f32 = flags & PTQ2_FP32, mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
   const uint8_t *mask = mm ? c->mask : nullptr;
-  const double *du = u, *dp = p;
-  double *dyu = yu, *dyp = yp;
+  const size_t es = f32 ? sizeof(float) : sizeof(double);
+  if (f32 && (rc = ptq2_internal_build_fac32(c, true))) return rc;
+  const void *du = u, *dp = p;
+  void *dyu = yu, *dyp = yp;
   if (!dev) {
     if ((rc = clear_err(c))) return rc;
-    if ((rc = stage_in<double>(c, u, &c->su, &c->su_bytes, c->nvel, true))) return rc;
-    if ((rc = stage_in<double>(c, p, (void **)&c->sp, &c->sp_bytes, c->npres, true))) return rc;
-    if ((rc = ensure(c, &c->sy, &c->sy_bytes, sizeof(double) * (size_t)c->nvel))) return rc;
-    if ((rc = ensure(c, (void **)&c->syp, &c->syp_bytes, sizeof(double) * (size_t)c->npres))) return rc;
-    if (add) {
-      HIPCHK(c, hipMemcpyAsync(c->sy, yu, sizeof(double) * (size_t)c->nvel, hipMemcpyHostToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->syp, yp, sizeof(double) * (size_t)c->npres, hipMemcpyHostToDevice, c->stream));
+    if (f32) {
+      if ((rc = stage_in<float>(c, u, &c->su, &c->su_bytes, c->nvel, true))) return rc;
+      if ((rc = stage_in<float>(c, p, (void **)&c->sp, &c->sp_bytes, c->npres, true))) return rc;
+    } else {
+      if ((rc = stage_in<double>(c, u, &c->su, &c->su_bytes, c->nvel, true))) return rc;
+      if ((rc = stage_in<double>(c, p, (void **)&c->sp, &c->sp_bytes, c->npres, true))) return rc;
     }
-    du = (const double *)c->su; dp = c->sp; dyu = (double *)c->sy; dyp = c->syp;
+    if ((rc = ensure(c, &c->sy, &c->sy_bytes, es * (size_t)c->nvel))) return rc;
+    if ((rc = ensure(c, (void **)&c->syp, &c->syp_bytes, es * (size_t)c->npres))) return rc;
+    if (add) {
+      HIPCHK(c, hipMemcpyAsync(c->sy, yu, es * (size_t)c->nvel, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->syp, yp, es * (size_t)c->npres, hipMemcpyHostToDevice, c->stream));
+    }
+    du = c->su; dp = c->sp; dyu = c->sy; dyp = c->syp;
   }
-  // momentum A11 u + A12 p, continuity A21 u (MFStokesWrapper_A_AVX in three passes)
-  HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, du, mask, c->eta_bm, dyu, add, c->stream));
-  HIPCHK(c, ptq2::launch_a12(c->box, c->coords, dp, dyu, true, c->stream));
-  HIPCHK(c, ptq2::launch_a21(c->box, c->coords, du, mask, dyp, add, c->stream));
-  if (mm) HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, du, dyu, 0, c->stream));
+  if (f32) {
+    // one fused pass (MFStokesWrapper_A_AVX's single element loop), FP32 factors
+    HIPCHK(c, ptq2::launch_stokes_fp32(c->box, (const float *)du, (const float *)dp, mask, c->fac32, c->facb32,
+                                       (float *)dyu, (float *)dyp, add, c->stream));
+    if (mm) HIPCHK(c, ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, (const float *)du, (float *)dyu, 0, c->stream));
+  } else {
+    // momentum A11 u + A12 p, continuity A21 u (MFStokesWrapper_A_AVX in three passes)
+    HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dyu, add,
+                                    c->stream));
+    HIPCHK(c, ptq2::launch_a12(c->box, c->coords, (const double *)dp, (double *)dyu, true, c->stream));
+    HIPCHK(c, ptq2::launch_a21(c->box, c->coords, (const double *)du, mask, (double *)dyp, add, c->stream));
+    if (mm)
+      HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, (const double *)du, (double *)dyu, 0, c->stream));
+  }
   if (!dev) {
-    HIPCHK(c, hipMemcpyAsync(yu, dyu, sizeof(double) * (size_t)c->nvel, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(yp, dyp, sizeof(double) * (size_t)c->npres, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(yu, dyu, es * (size_t)c->nvel, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(yp, dyp, es * (size_t)c->npres, hipMemcpyDeviceToHost, c->stream));
     int bits = 0;
     if ((rc = read_err(c, &bits))) return rc;
     if (bits & ptq2::ERRBIT_NONFINITE) return fail(c, PTQ2_ERR_NONFINITE, "non-finite value in the input vector");

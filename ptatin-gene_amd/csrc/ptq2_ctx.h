@@ -21,6 +21,8 @@ struct ptq2_ctx {
   double *eta_bm = nullptr;
   float *fac32 = nullptr;
   bool fac32_valid = false;
+  float *facb32 = nullptr;  // FP32 full-block factors (w det / eta scale, pressure basis)
+  bool facb32_valid = false;
   bool has_state = false;
   uint64_t state = 0;
   int *err = nullptr;
@@ -33,5 +35,6 @@ struct ptq2_ctx {
 };
 
 
-// (re)build the FP32 geometric factors of the smoother operator if stale
-int ptq2_internal_build_fac32(ptq2_ctx *c);
+// (re)build the FP32 geometric factors of the smoother operator if stale; with block,
+// also those of the FP32 full block
+int ptq2_internal_build_fac32(ptq2_ctx *c, bool block = false);

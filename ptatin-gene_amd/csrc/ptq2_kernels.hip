@@ -149,10 +149,29 @@ __device__ __forceinline__ void a11_qp32(const float ur0[3][3], const float ur1[
   acc_row<qa>(dv, P0, P1, P2);
 }
 
-// the 27 factors of quadrature row `row` (brick-major, lane-minor: coalesced)
-__device__ __forceinline__ void load_fac_row(const float *__restrict__ fbase, int row, float F[27]) {
+// z-contraction of the velocity tile for one slab (runtime qc): for every column
+// (c,i,j) the B- and D-weighted sums over the three z nodes.  The three rows of the slab
+// reuse them, so the tile is read once per slab instead of once per row.
+__device__ __forceinline__ void slab_z32(const float *Us, int base, const Coef1 &cc, float ZB[3][3][3],
+                                         float ZD[3][3][3]) {
 #pragma unroll
-  for (int n = 0; n < 27; n++) F[n] = fbase[(int64_t)(27 * row + n) * BEL];
+  for (int c = 0; c < 3; c++) {
+    float v[3][3][3];
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) v[k][j][i] = tv(Us, base, c, i, j, k);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        ZB[c][i][j] = fmac(cc.b[2], v[2][j][i], fmac(cc.b[1], v[1][j][i], static_cast<float>(cc.b[0]) * v[0][j][i]));
+        ZD[c][i][j] = fmac(cc.d[2], v[2][j][i], fmac(cc.d[1], v[1][j][i], static_cast<float>(cc.d[0]) * v[0][j][i]));
+      }
+    __builtin_amdgcn_sched_barrier(0);
+  }
 }
 
 __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
@@ -163,8 +182,6 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   const BrickId br = brick_of(b, color, blockIdx.x);
   const Tile t = tile_of(b, br);
   const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
-  float Fn[27];
-  load_fac_row(fbase, 0, Fn);
   load_tile(b, t, br, u, mask, Us, lane);
   __syncthreads();
   const int base = lane_base(lane);
@@ -179,27 +196,39 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
   if (active) {
-    float ca[9], cb9[9];
-    ld_col(Us, base, 0, ca);
+    // the first point's factors of row r+1 are loaded while row r's outputs are
+    // accumulated; the other two points' factors at the start of their row
+    float F0[9];
+#pragma unroll
+    for (int n = 0; n < 9; n++) F0[n] = fbase[(int64_t)n * BEL];
+    float ZB[3][3][3], ZD[3][3][3];
 #pragma nounroll
     for (int row = 0; row < 9; row++) {
       const int qb = row % 3, qc = row / 3;
       const Coef1 cb = coef1(qb), cc = coef1(qc);
-      float F[27];
+      float F12[18];
 #pragma unroll
-      for (int n = 0; n < 27; n++) F[n] = Fn[n];
-      if (row < 8) load_fac_row(fbase, row + 1, Fn);
+      for (int n = 0; n < 18; n++) F12[n] = fbase[(int64_t)(27 * row + 9 + n) * BEL];
+      if (qb == 0) slab_z32(Us, base, cc, ZB, ZD);
       float ur0[3][3], ur1[3][3], ur2[3][3];
-      // column 0 for the next row lands in cb9; swap roles by copying back (9 moves)
-      row_data_pipe(Us, Us, base, cb, cc, ca, cb9, ur0, ur1, ur2);
 #pragma unroll
-      for (int n = 0; n < 9; n++) ca[n] = cb9[n];
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          ur0[c][i] = fmac(cb.b[2], ZB[c][i][2], fmac(cb.b[1], ZB[c][i][1], static_cast<float>(cb.b[0]) * ZB[c][i][0]));
+          ur1[c][i] = fmac(cb.d[2], ZB[c][i][2], fmac(cb.d[1], ZB[c][i][1], static_cast<float>(cb.d[0]) * ZB[c][i][0]));
+          ur2[c][i] = fmac(cb.b[2], ZD[c][i][2], fmac(cb.b[1], ZD[c][i][1], static_cast<float>(cb.b[0]) * ZD[c][i][0]));
+        }
       float P0[3][3], P1[3][3], P2[3][3];
       zero33(P0); zero33(P1); zero33(P2);
-      a11_qp32<0>(ur0, ur1, ur2, F, P0, P1, P2);
-      a11_qp32<1>(ur0, ur1, ur2, F + 9, P0, P1, P2);
-      a11_qp32<2>(ur0, ur1, ur2, F + 18, P0, P1, P2);
+      a11_qp32<0>(ur0, ur1, ur2, F0, P0, P1, P2);
+      a11_qp32<1>(ur0, ur1, ur2, F12, P0, P1, P2);
+      a11_qp32<2>(ur0, ur1, ur2, F12 + 9, P0, P1, P2);
       row_fence();
+      if (row < 8) {
+#pragma unroll
+        for (int n = 0; n < 9; n++) F0[n] = fbase[(int64_t)(27 * (row + 1) + n) * BEL];
+      }
       acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
       row_fence();
     }
@@ -212,10 +241,147 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   store_tile(b, br, t, Us, y, add_mode != 0, lane);
 }
 
+// ------------------------------------------------- FP32 full block [A11 A12; A21 0]
+// One pass: the A11 point action of k_a11_fp32 plus, per point, p_q = pe . N^p and
+//   dv[k][l] -= p_q w adj(J)[l][k]            (A12, QuadratureAction_A12_AVX)
+//   yp_i     -= N^p_i w trace(adj(J) grad u)  (A21, MatMultMF_Stokes_MixedFEM3d_A21)
+// with w adj(J) = facb0 F taken from the FP32 factors (MFStokesWrapper_A_AVX,
+// src/stokes_operators_avx.c:439-636, in FP32 with FP32-stored geometry).
+template <int qa>
+__device__ __forceinline__ void stokes_qp32(const float ur0[3][3], const float ur1[3][3], const float ur2[3][3],
+                                            const float F9[9], const float B4[4], const float pe[4], float P0[3][3],
+                                            float P1[3][3], float P2[3][3], float Yp[4]) {
+  float g[3][3], dv[3][3];
+  qp_grad<qa>(ur0, ur1, ur2, g);
+  float M[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) M[k][l] = g[0][l] * F9[3 * k] + g[1][l] * F9[3 * k + 1] + g[2][l] * F9[3 * k + 2];
+  float S[3][3];
+  S[0][0] = 2.0f * M[0][0];
+  S[1][1] = 2.0f * M[1][1];
+  S[2][2] = 2.0f * M[2][2];
+  S[0][1] = S[1][0] = M[0][1] + M[1][0];
+  S[0][2] = S[2][0] = M[0][2] + M[2][0];
+  S[1][2] = S[2][1] = M[1][2] + M[2][1];
+  const float pq = pe[0] + pe[1] * B4[1] + pe[2] * B4[2] + pe[3] * B4[3];
+  const float fp = -pq * B4[0];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+      dv[k][l] = S[0][l] * F9[k] + S[1][l] * F9[3 + k] + S[2][l] * F9[6 + k] + fp * F9[3 * l + k];
+  const float f = -B4[0] * (M[0][0] + M[1][1] + M[2][2]);
+  Yp[0] += f;
+  Yp[1] += f * B4[1];
+  Yp[2] += f * B4[2];
+  Yp[3] += f * B4[3];
+  acc_row<qa>(dv, P0, P1, P2);
+}
+
+__global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const float *__restrict__ u,
+                                                     const float *__restrict__ p, const uint8_t *__restrict__ mask,
+                                                     const float *__restrict__ fac, const float *__restrict__ facb,
+                                                     float *__restrict__ yu, float *__restrict__ yp, int add_mode) {
+  __shared__ float Us[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_of(b, color, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
+  const float *bbase = facb + (int64_t)br.lin * 108 * BEL + lane;
+  load_tile(b, t, br, u, mask, Us, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  const int64_t e = lane_element(b, t, lane);
+  float Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
+  float Yp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+  if (active) {
+    const float pe[4] = {p[4 * e], p[4 * e + 1], p[4 * e + 2], p[4 * e + 3]};
+    float ZB[3][3][3], ZD[3][3][3];
+#pragma nounroll
+    for (int row = 0; row < 9; row++) {
+      const int qb = row % 3, qc = row / 3;
+      const Coef1 cb = coef1(qb), cc = coef1(qc);
+      float F[27], B4[12];
+#pragma unroll
+      for (int n = 0; n < 27; n++) F[n] = fbase[(int64_t)(27 * row + n) * BEL];
+#pragma unroll
+      for (int n = 0; n < 12; n++) B4[n] = bbase[(int64_t)(12 * row + n) * BEL];
+      if (qb == 0) slab_z32(Us, base, cc, ZB, ZD);
+      float ur0[3][3], ur1[3][3], ur2[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          ur0[c][i] = fmac(cb.b[2], ZB[c][i][2], fmac(cb.b[1], ZB[c][i][1], static_cast<float>(cb.b[0]) * ZB[c][i][0]));
+          ur1[c][i] = fmac(cb.d[2], ZB[c][i][2], fmac(cb.d[1], ZB[c][i][1], static_cast<float>(cb.d[0]) * ZB[c][i][0]));
+          ur2[c][i] = fmac(cb.b[2], ZD[c][i][2], fmac(cb.b[1], ZD[c][i][1], static_cast<float>(cb.b[0]) * ZD[c][i][0]));
+        }
+      float P0[3][3], P1[3][3], P2[3][3];
+      zero33(P0); zero33(P1); zero33(P2);
+      stokes_qp32<0>(ur0, ur1, ur2, F, B4, pe, P0, P1, P2, Yp);
+      stokes_qp32<1>(ur0, ur1, ur2, F + 9, B4 + 4, pe, P0, P1, P2, Yp);
+      stokes_qp32<2>(ur0, ur1, ur2, F + 18, B4 + 8, pe, P0, P1, P2, Yp);
+      row_fence();
+      acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
+      row_fence();
+    }
+    float *o = yp + 4 * e;
+#pragma unroll
+    for (int i = 0; i < 4; i++) o[i] = add_mode ? o[i] + Yp[i] : Yp[i];
+  }
+  __syncthreads();
+  zero_tile(Us, lane);
+  __syncthreads();
+  if (active) emit_element(Us, base, Y);
+  __syncthreads();
+  store_tile(b, br, t, Us, yu, add_mode != 0, lane);
+}
+
+// ------------------------------------------------------- pressure basis helper
+// P3D_ConstructNi_P1GRel_3D (src/element_utils_q2.c:183-224):
+// N^p = [1, (x-xbar)/Lx, (y-ybar)/Ly, (z-zbar)/Lz], xbar the mean of the 27 nodes,
+// Lx = x_14 - x_12, Ly = y_16 - y_10, Lz = z_22 - z_4.
+struct PBasis {
+  double avg[3], inv[3];
+};
+__device__ __forceinline__ PBasis pbasis_setup(const double *Xs, int base) {
+  PBasis pb;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) s += tv(Xs, base, c, i, j, k);
+    pb.avg[c] = s * (1.0 / 27.0);
+  }
+  pb.inv[0] = 1.0 / (tv(Xs, base, 0, 2, 1, 1) - tv(Xs, base, 0, 0, 1, 1));
+  pb.inv[1] = 1.0 / (tv(Xs, base, 1, 1, 2, 1) - tv(Xs, base, 1, 1, 0, 1));
+  pb.inv[2] = 1.0 / (tv(Xs, base, 2, 1, 1, 2) - tv(Xs, base, 2, 1, 1, 0));
+  return pb;
+}
+
 // FP64 geometry -> FP32 factors, with validation of det > 0 and eta > 0.
+//   fac  [brick][q][9][lane]: F = sqrt(w eta / det) adj(J)
+//   facb [brick][q][4][lane] (optional, for the FP32 full block): sqrt(w det / eta) (so that
+//        w adj(J) = facb0 * F) and the P1GRel pressure basis values N1..N3 at the point.
 template <int qa, int qb, int qc>
 __device__ __forceinline__ void fac_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
-                                       const double *eta27, float *__restrict__ fp, int &bad) {
+                                       const double *eta27, const PBasis *pb, float *__restrict__ fp,
+                                       float *__restrict__ fbp, int &bad) {
   double J[3][3], A[3][3];
   qp_grad<qa>(xr0, xr1, xr2, J);
   double det = adjugate(J, A);
@@ -227,21 +393,29 @@ __device__ __forceinline__ void fac_qp(const double xr0[3][3], const double xr1[
   double s = (bad == 0) ? sqrt(w * et / det) : 0.0;
 #pragma unroll
   for (int m = 0; m < 9; m++) fp[(q * 9 + m) * BEL] = (float)(s * A[m / 3][m % 3]);
+  if (fbp) {
+    double x[3];
+    qp_value<qa>(xr0, x);
+    fbp[(q * 4) * BEL] = (float)((bad == 0) ? sqrt(w * det / et) : 0.0);
+#pragma unroll
+    for (int d = 0; d < 3; d++) fbp[(q * 4 + 1 + d) * BEL] = (float)((x[d] - pb->avg[d]) * pb->inv[d]);
+  }
 }
 
 template <int qb, int qc>
-__device__ __forceinline__ void fac_row(const double *Xs, int base, const double *eta27, float *fp, int &bad) {
+__device__ __forceinline__ void fac_row(const double *Xs, int base, const double *eta27, const PBasis *pb, float *fp,
+                                        float *fbp, int &bad) {
   double xr0[3][3], xr1[3][3], xr2[3][3];
   row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
-  fac_qp<0, qb, qc>(xr0, xr1, xr2, eta27, fp, bad);
-  fac_qp<1, qb, qc>(xr0, xr1, xr2, eta27, fp, bad);
-  fac_qp<2, qb, qc>(xr0, xr1, xr2, eta27, fp, bad);
+  fac_qp<0, qb, qc>(xr0, xr1, xr2, eta27, pb, fp, fbp, bad);
+  fac_qp<1, qb, qc>(xr0, xr1, xr2, eta27, pb, fp, fbp, bad);
+  fac_qp<2, qb, qc>(xr0, xr1, xr2, eta27, pb, fp, fbp, bad);
   row_fence();
 }
 
 __global__ __launch_bounds__(BEL) void k_build_fac32(Box b, const double *__restrict__ coords,
                                                      const double *__restrict__ eta_bm, float *__restrict__ fac,
-                                                     int *__restrict__ err) {
+                                                     float *__restrict__ facb, int *__restrict__ err) {
   __shared__ double Xs[3 * TCS];
   const int lane = threadIdx.x;
   const BrickId br = brick_lin(b, blockIdx.x);
@@ -255,10 +429,16 @@ __global__ __launch_bounds__(BEL) void k_build_fac32(Box b, const double *__rest
 #pragma unroll
   for (int q = 0; q < 27; q++) eta27[q] = ep[q * BEL];
   float *fp = fac + (int64_t)br.lin * 243 * BEL + lane;
+  float *fbp = facb ? facb + (int64_t)br.lin * 108 * BEL + lane : nullptr;
+  PBasis pbv;
+  if (facb) pbv = pbasis_setup(Xs, base);
+  const PBasis *pb = &pbv;
   int bad = 0;
-  fac_row<0, 0>(Xs, base, eta27, fp, bad); fac_row<1, 0>(Xs, base, eta27, fp, bad); fac_row<2, 0>(Xs, base, eta27, fp, bad);
-  fac_row<0, 1>(Xs, base, eta27, fp, bad); fac_row<1, 1>(Xs, base, eta27, fp, bad); fac_row<2, 1>(Xs, base, eta27, fp, bad);
-  fac_row<0, 2>(Xs, base, eta27, fp, bad); fac_row<1, 2>(Xs, base, eta27, fp, bad); fac_row<2, 2>(Xs, base, eta27, fp, bad);
+  fac_row<0, 0>(Xs, base, eta27, pb, fp, fbp, bad); fac_row<1, 0>(Xs, base, eta27, pb, fp, fbp, bad);
+  fac_row<2, 0>(Xs, base, eta27, pb, fp, fbp, bad); fac_row<0, 1>(Xs, base, eta27, pb, fp, fbp, bad);
+  fac_row<1, 1>(Xs, base, eta27, pb, fp, fbp, bad); fac_row<2, 1>(Xs, base, eta27, pb, fp, fbp, bad);
+  fac_row<0, 2>(Xs, base, eta27, pb, fp, fbp, bad); fac_row<1, 2>(Xs, base, eta27, pb, fp, fbp, bad);
+  fac_row<2, 2>(Xs, base, eta27, pb, fp, fbp, bad);
   if (bad & 1) atomicOr(err, ERRBIT_MESH);
   if (bad & 2) atomicOr(err, ERRBIT_VISCOSITY);
 }
@@ -312,32 +492,6 @@ __global__ __launch_bounds__(BEL) void k_check_mesh(Box b, const double *__restr
   check_row<0, 1>(Xs, base, bad); check_row<1, 1>(Xs, base, bad); check_row<2, 1>(Xs, base, bad);
   check_row<0, 2>(Xs, base, bad); check_row<1, 2>(Xs, base, bad); check_row<2, 2>(Xs, base, bad);
   if (bad) atomicOr(err, ERRBIT_MESH);
-}
-
-// ------------------------------------------------------- pressure basis helper
-// P3D_ConstructNi_P1GRel_3D (src/element_utils_q2.c:183-224):
-// N^p = [1, (x-xbar)/Lx, (y-ybar)/Ly, (z-zbar)/Lz], xbar the mean of the 27 nodes,
-// Lx = x_14 - x_12, Ly = y_16 - y_10, Lz = z_22 - z_4.
-struct PBasis {
-  double avg[3], inv[3];
-};
-__device__ __forceinline__ PBasis pbasis_setup(const double *Xs, int base) {
-  PBasis pb;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) s += tv(Xs, base, c, i, j, k);
-    pb.avg[c] = s * (1.0 / 27.0);
-  }
-  pb.inv[0] = 1.0 / (tv(Xs, base, 0, 2, 1, 1) - tv(Xs, base, 0, 0, 1, 1));
-  pb.inv[1] = 1.0 / (tv(Xs, base, 1, 1, 2, 1) - tv(Xs, base, 1, 1, 0, 1));
-  pb.inv[2] = 1.0 / (tv(Xs, base, 2, 1, 1, 2) - tv(Xs, base, 2, 1, 1, 0));
-  return pb;
 }
 
 // ------------------------------------------------------------------ A21
@@ -613,9 +767,19 @@ hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, co
   return hipGetLastError();
 }
 
-hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, int *err,
-                              hipStream_t s) {
-  hipLaunchKernelGGL(k_build_fac32, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, eta_bm, fac, err);
+hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, float *facb,
+                              int *err, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_fac32, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, eta_bm, fac, facb, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_stokes_fp32(const Box &b, const float *u, const float *p, const uint8_t *mask, const float *fac,
+                              const float *facb, float *yu, float *yp, bool add, hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_stokes_fp32, dim3(nb), dim3(BEL), 0, s, b, c, u, p, mask, fac, facb, yu, yp, add ? 1 : 0);
+  }
   return hipGetLastError();
 }
 

@@ -138,11 +138,12 @@ __global__ void k_prolong_add(int NXc, int NYc, int NXf, int NYf, int NZf, const
 
 // dense coarse solve x = Ainv b (FP64 inverse, one wave per row)
 template <typename T>
-__global__ void k_dense_gemv(int n, const double *__restrict__ Ainv, const T *__restrict__ b, T *__restrict__ x) {
+__global__ void k_dense_gemv(int n, int ld, const double *__restrict__ Ainv, const T *__restrict__ b,
+                             T *__restrict__ x) {
   int row = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
   int lane = threadIdx.x & 63;
   if (row >= n) return;
-  const double *a = Ainv + (int64_t)row * n;
+  const double *a = Ainv + (int64_t)row * ld;
   double s = 0.0;
   for (int j = lane; j < n; j += 64) s += a[j] * (double)b[j];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
@@ -291,6 +292,7 @@ struct ptq2_mg {
   std::vector<void *> vb, vx, vy, vp1, vp2, vdinv;  // level vectors in the smoothing precision
   std::vector<double> emax, emin;
   double *ainv = nullptr;               // dense inverse of the coarsest operator
+  int ainv_ld = 0;                      // its leading dimension (padded to the sweep block)
   double *r_in = nullptr, *z_out = nullptr;  // FP64 graph input/output (level 0)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -394,7 +396,7 @@ int vcycle_level(ptq2_mg *mg, int l) {
   T *b = (T *)mg->vb[l], *x = (T *)mg->vx[l], *y = (T *)mg->vy[l];
   if (l == mg->nl - 1) {
     int nn = (int)n;
-    hipLaunchKernelGGL(k_dense_gemv<T>, dim3((nn + 3) / 4), dim3(256), 0, mg->stream, nn, (const double *)mg->ainv,
+    hipLaunchKernelGGL(k_dense_gemv<T>, dim3((nn + 3) / 4), dim3(256), 0, mg->stream, nn, mg->ainv_ld, (const double *)mg->ainv,
                        (const T *)b, x);
     return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "coarse solve launch");
   }
@@ -455,42 +457,149 @@ int norm2(ptq2_mg *mg, const double *x, int64_t n, double *out) {
 }
 
 // dense Cholesky inverse of an SPD matrix (row-major, n x n), in place
-bool spd_inverse(std::vector<double> &A, int n) {
-  std::vector<double> L(A);
-  for (int j = 0; j < n; j++) {
-    double *Lj = &L[(size_t)j * n];
-    double d = Lj[j];
-    for (int k = 0; k < j; k++) d -= Lj[k] * Lj[k];
-    if (!(d > 0.0)) return false;
-    d = std::sqrt(d);
-    Lj[j] = d;
-    for (int i = j + 1; i < n; i++) {
-      double *Li = &L[(size_t)i * n];
-      double s = Li[j];
-      for (int k = 0; k < j; k++) s -= Li[k] * Lj[k];
-      Li[j] = s / d;
-    }
-  }
-  // inverse: solve L L^T X = I column by column
-  std::vector<double> col(n);
-  for (int c = 0; c < n; c++) {
-    std::fill(col.begin(), col.end(), 0.0);
-    col[c] = 1.0;
-    for (int i = 0; i < n; i++) {
-      const double *Li = &L[(size_t)i * n];
-      double s = col[i];
-      for (int k = 0; k < i; k++) s -= Li[k] * col[k];
-      col[i] = s / Li[i];
-    }
-    for (int i = n - 1; i >= 0; i--) {
-      double s = col[i];
-      for (int k = i + 1; k < n; k++) s -= L[(size_t)k * n + i] * col[k];
-      col[i] = s / L[(size_t)i * n + i];
-    }
-    for (int i = 0; i < n; i++) A[(size_t)i * n + c] = col[i];
-  }
-  return true;
+// ---------------------------------------------------------------- coarse inverse
+// Dense inverse of the coarsest operator by the block sweep (Gauss-Jordan elimination
+// without pivoting, valid for SPD matrices): sweeping diagonal block k with D = A_kk^-1
+// maps A_kk -> -D, A_ik -> A_ik D, A_kj -> D A_kj, A_ij -> A_ij - A_ik D A_kj; after all
+// blocks the matrix holds -A^-1.  Blocks of SB = 64; the matrix is padded to a multiple of
+// SB with identity rows.  Every output has one writer and a fixed summation order, so the
+// result is deterministic.
+constexpr int SB = 64;
+
+// column j of the operator assembled into row j of A (ld columns): e = unit vector j
+__global__ void k_unit(int64_t n, int64_t j, double *__restrict__ e) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) e[i] = i == j ? 1.0 : 0.0;
 }
+__global__ void k_pad_identity(int nc, int np, double *__restrict__ A) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)np * np) return;
+  int i = (int)(idx / np), j = (int)(idx % np);
+  if (i >= nc || j >= nc) A[idx] = i == j ? 1.0 : 0.0;
+}
+// symmetrise the round-off of the assembled operator
+__global__ void k_symmetrise(int np, double *__restrict__ A) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)np * np) return;
+  int i = (int)(idx / np), j = (int)(idx % np);
+  if (i < j) {
+    double s = 0.5 * (A[idx] + A[(int64_t)j * np + i]);
+    A[idx] = s;
+    A[(int64_t)j * np + i] = s;
+  }
+}
+// sweep of the SB x SB diagonal block k in LDS: Dm = -A_kk^-1; flags a non-positive pivot
+__global__ __launch_bounds__(256) void k_sweep_diag(int np, int k, const double *__restrict__ A,
+                                                    double *__restrict__ Dm, int *__restrict__ err) {
+  __shared__ double S[SB][SB + 1];
+  __shared__ double piv_s;
+  const int t = threadIdx.x;
+  const int64_t o = (int64_t)k * SB;
+  for (int e = t; e < SB * SB; e += 256) S[e / SB][e % SB] = A[(o + e / SB) * np + o + e % SB];
+  __syncthreads();
+  for (int p = 0; p < SB; p++) {
+    if (t == 0) {
+      piv_s = S[p][p];
+      if (!(piv_s > 0.0)) atomicOr(err, 1);
+    }
+    __syncthreads();
+    const double piv = piv_s;
+    const double rp = piv > 0.0 ? 1.0 / piv : 0.0;
+    // a_ij -= a_ip a_pj / a_pp for i,j != p (reads row p and column p, which stay unchanged here)
+    for (int e = t; e < SB * SB; e += 256) {
+      const int i = e / SB, j = e % SB;
+      if (i != p && j != p) S[i][j] -= S[i][p] * S[p][j] * rp;
+    }
+    __syncthreads();
+    for (int e = t; e < SB; e += 256)
+      if (e != p) {
+        S[e][p] *= rp;
+        S[p][e] *= rp;
+      }
+    if (t == 0) S[p][p] = -rp;
+    __syncthreads();
+  }
+  for (int e = t; e < SB * SB; e += 256) Dm[e] = S[e / SB][e % SB];
+}
+// panel of block column k: C_i = A_ik (old), P_i = A_ik D = -A_ik Dm for every block row i != k
+__global__ __launch_bounds__(256) void k_sweep_panel(int np, int k, const double *__restrict__ A,
+                                                     const double *__restrict__ Dm, double *__restrict__ P,
+                                                     double *__restrict__ C) {
+  __shared__ double Ds[SB][SB + 1];
+  __shared__ double As[4][SB];
+  const int t = threadIdx.x;
+  for (int e = t; e < SB * SB; e += 256) Ds[e / SB][e % SB] = Dm[e];
+  const int r0 = blockIdx.x * 4;  // 4 rows per block, 64 threads per row
+  const int rl = t / SB, col = t % SB;
+  const int64_t row = r0 + rl;
+  const int64_t o = (int64_t)k * SB;
+  const bool live = row < np && row / SB != k;
+  double a = 0.0;
+  if (live) a = A[row * np + o + col];
+  As[rl][col] = a;
+  __syncthreads();
+  if (!live) return;
+  double s = 0.0;
+  for (int q = 0; q < SB; q++) s = fma(As[rl][q], Ds[q][col], s);
+  C[row * SB + col] = a;
+  P[row * SB + col] = -s;
+}
+// rank-SB update of every tile outside block row/column k, and the new block row/column k
+__global__ __launch_bounds__(256) void k_sweep_update(int np, int k, double *__restrict__ A,
+                                                      const double *__restrict__ P, const double *__restrict__ C,
+                                                      const double *__restrict__ Dm) {
+  __shared__ double Ps[SB][SB + 1];
+  __shared__ double Cs[SB][SB + 1];
+  const int bi = blockIdx.y, bj = blockIdx.x;
+  const int t = threadIdx.x;
+  const int64_t i0 = (int64_t)bi * SB, j0 = (int64_t)bj * SB;
+  if (bi == k || bj == k) {
+    for (int e = t; e < SB * SB; e += 256) {
+      const int a = e / SB, c = e % SB;
+      double v;
+      if (bi == k && bj == k)
+        v = Dm[e];
+      else if (bj == k)
+        v = P[(i0 + a) * SB + c];  // A_ik' = P_i
+      else
+        v = P[(j0 + c) * SB + a];  // A_kj' = P_j^T
+      A[(i0 + a) * np + j0 + c] = v;
+    }
+    return;
+  }
+  for (int e = t; e < SB * SB; e += 256) {
+    Ps[e / SB][e % SB] = P[(i0 + e / SB) * SB + e % SB];
+    Cs[e / SB][e % SB] = C[(j0 + e / SB) * SB + e % SB];
+  }
+  __syncthreads();
+  // thread owns a 4x4 sub-tile: rows 4*(t/16).., cols 4*(t%16)..
+  const int ra = 4 * (t / 16), ca = 4 * (t % 16);
+  double acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) acc[a][c] = A[(i0 + ra + a) * np + j0 + ca + c];
+  for (int q = 0; q < SB; q++) {
+    double pa[4], cc[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) pa[a] = Ps[ra + a][q];
+#pragma unroll
+    for (int c = 0; c < 4; c++) cc[c] = Cs[ca + c][q];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+      for (int c = 0; c < 4; c++) acc[a][c] = fma(-pa[a], cc[c], acc[a][c]);
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int c = 0; c < 4; c++) A[(i0 + ra + a) * np + j0 + ca + c] = acc[a][c];
+}
+__global__ void k_negate(int64_t n, double *__restrict__ x) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = -x[i];
+}
+
 
 }  // namespace
 
@@ -505,7 +614,7 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->emax_frac = 1.1;
   o->esteig_its = 20;
   o->use_graph = 1;
-  o->coarse_max_dofs = 6000;
+  o->coarse_max_dofs = 20000;
 }
 
 int ptq2_mg_destroy(ptq2_mg **pmg) {
@@ -682,37 +791,47 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     (void)hipFree(d); (void)hipFree(v); (void)hipFree(w);
     MGRC(rc);
   }
-  // ---- dense coarse inverse
+  // ---- dense coarse inverse, assembled and inverted on the device
   {
     const int l = mg->nl - 1;
     ptq2_ctx *c = mg->ctx[l];
     const int nc = (int)mg->n[l];
-    std::vector<double> A((size_t)nc * nc);
-    double *e = nullptr, *y = nullptr;
+    const int np = (nc + SB - 1) / SB * SB;
+    const int nbk = np / SB;
+    const int64_t nn = (int64_t)np * np;
+    double *e = nullptr, *P = nullptr, *C = nullptr, *Dm = nullptr;
+    int *err = nullptr;
+    MGRC(mg_alloc(mg, (void **)&mg->ainv, sizeof(double) * (size_t)nn));
+    mg->ainv_ld = np;
     MGRC(mg_alloc(mg, (void **)&e, sizeof(double) * nc));
-    MGRC(mg_alloc(mg, (void **)&y, sizeof(double) * nc));
-    std::vector<double> col((size_t)nc);
     int rc = PTQ2_OK;
+    hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, nc, np,
+                       mg->ainv);
     for (int j = 0; j < nc && rc == PTQ2_OK; j++) {
-      std::fill(col.begin(), col.end(), 0.0);
-      col[j] = 1.0;
-      (void)hipMemcpyAsync(e, col.data(), sizeof(double) * nc, hipMemcpyHostToDevice, mg->stream);
-      rc = ptq2_a11_apply(c, e, y, PTQ2_DEVICE | PTQ2_MATMULT);
-      (void)hipMemcpyAsync(col.data(), y, sizeof(double) * nc, hipMemcpyDeviceToHost, mg->stream);
-      (void)hipStreamSynchronize(mg->stream);
-      for (int i = 0; i < nc; i++) A[(size_t)i * nc + j] = col[i];
+      hipLaunchKernelGGL(k_unit, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, (int64_t)nc, (int64_t)j, e);
+      rc = ptq2_a11_apply(c, e, mg->ainv + (int64_t)j * np, PTQ2_DEVICE | PTQ2_MATMULT);
     }
-    (void)hipFree(e); (void)hipFree(y);
+    (void)hipFree(e);
     MGRC(rc);
-    // symmetrise the round-off before factorising
-    for (int i = 0; i < nc; i++)
-      for (int j = i + 1; j < nc; j++) {
-        double s = 0.5 * (A[(size_t)i * nc + j] + A[(size_t)j * nc + i]);
-        A[(size_t)i * nc + j] = A[(size_t)j * nc + i] = s;
-      }
-    if (!spd_inverse(A, nc)) return mgfail(mg, PTQ2_ERR_MESH, "coarse operator is not positive definite");
-    MGRC(mg_alloc(mg, (void **)&mg->ainv, sizeof(double) * (size_t)nc * nc));
-    MGCHK(hipMemcpy(mg->ainv, A.data(), sizeof(double) * (size_t)nc * nc, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_symmetrise, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, np, mg->ainv);
+    MGRC(mg_alloc(mg, (void **)&P, sizeof(double) * (size_t)np * SB));
+    MGRC(mg_alloc(mg, (void **)&C, sizeof(double) * (size_t)np * SB));
+    MGRC(mg_alloc(mg, (void **)&Dm, sizeof(double) * SB * SB));
+    MGRC(mg_alloc(mg, (void **)&err, sizeof(int)));
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(TPB), 0, mg->stream, (int64_t)1, err, 0);
+    for (int k = 0; k < nbk; k++) {
+      hipLaunchKernelGGL(k_sweep_diag, dim3(1), dim3(256), 0, mg->stream, np, k, (const double *)mg->ainv, Dm, err);
+      hipLaunchKernelGGL(k_sweep_panel, dim3((np + 3) / 4), dim3(256), 0, mg->stream, np, k, (const double *)mg->ainv,
+                         (const double *)Dm, P, C);
+      hipLaunchKernelGGL(k_sweep_update, dim3(nbk, nbk), dim3(256), 0, mg->stream, np, k, mg->ainv,
+                         (const double *)P, (const double *)C, (const double *)Dm);
+    }
+    hipLaunchKernelGGL(k_negate, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, nn, mg->ainv);
+    int herr = 0;
+    MGCHK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, mg->stream));
+    MGCHK(hipStreamSynchronize(mg->stream));
+    (void)hipFree(P); (void)hipFree(C); (void)hipFree(Dm); (void)hipFree(err);
+    if (herr) return mgfail(mg, PTQ2_ERR_MESH, "coarse operator is not positive definite");
   }
   // ---- graph of one V-cycle
   MGRC(mg_alloc(mg, (void **)&mg->r_in, sizeof(double) * mg->n[0]));

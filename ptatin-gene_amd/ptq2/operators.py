@@ -140,22 +140,29 @@ class Context:
         return self._call1(L.ptq2_a21_apply, "ptq2_a21_apply", u, y, flags, np.float64, dt, self.npres)
 
     def stokes(self, u, p, yu=None, yp=None, flags=0):
+        """[A11 A12; A21 0] [u; p]; with FP32 in flags all vectors are float32."""
         L = _lib.lib()
+        f32 = bool(flags & FP32)
         if _is_dev(u):
+            dt = torch.float32 if f32 else torch.float64
+            for v in (u, p, yu, yp):
+                if v is not None and v.dtype != dt:
+                    raise TypeError("ptq2_stokes_apply: vectors must be %s" % dt)
             if yu is None:
-                yu = torch.zeros(self.nvel, dtype=torch.float64, device=u.device)
+                yu = torch.zeros(self.nvel, dtype=dt, device=u.device)
             if yp is None:
-                yp = torch.zeros(self.npres, dtype=torch.float64, device=u.device)
+                yp = torch.zeros(self.npres, dtype=dt, device=u.device)
             self._stream(u, p, yu, yp)
             check(L.ptq2_stokes_apply(self.handle, u.data_ptr(), p.data_ptr(), yu.data_ptr(), yp.data_ptr(),
                                       flags | DEVICE), "ptq2_stokes_apply", self.handle)
             return yu, yp
-        un = np.ascontiguousarray(u, dtype=np.float64)
-        pn = np.ascontiguousarray(p, dtype=np.float64)
+        dt = np.float32 if f32 else np.float64
+        un = np.ascontiguousarray(u, dtype=dt)
+        pn = np.ascontiguousarray(p, dtype=dt)
         if yu is None:
-            yu = np.zeros(self.nvel)
+            yu = np.zeros(self.nvel, dtype=dt)
         if yp is None:
-            yp = np.zeros(self.npres)
+            yp = np.zeros(self.npres, dtype=dt)
         check(L.ptq2_stokes_apply(self.handle, un.ctypes.data, pn.ctypes.data, yu.ctypes.data, yp.ctypes.data,
                                   flags), "ptq2_stokes_apply", self.handle)
         return yu, yp

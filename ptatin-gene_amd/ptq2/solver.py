@@ -56,7 +56,7 @@ def default_opts():
 class ViscousSolver:
     """GMG V-cycle (hipGraph) + FGMRES for MatMult_MFStokes_A11 x = b on one GPU."""
 
-    def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=6000):
+    def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000):
         L = _sl()
         o = default_opts()
         o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
@@ -127,3 +127,39 @@ class ViscousSolver:
             self.destroy()
         except Exception:
             pass
+
+
+def auto_levels(m, coarse_max_dofs=20000):
+    """Number of levels for an m^3 mesh coarsened by 2 until the coarse level has at most
+    coarse_max_dofs velocity dofs (or cannot be halved)."""
+    levels = 1
+    while m % 2 == 0 and 3 * (2 * m + 1) ** 3 > coarse_max_dofs:
+        m //= 2
+        levels += 1
+    return levels
+
+
+def solve_batch(solvers, rhs, rtol=1e-8, max_it=400, restart=30):
+    """Solve independent viscous problems concurrently: one host thread and one stream per
+    solver (the C solver releases the GIL inside ctypes), so the small kernels of the
+    different problems overlap on the GPU.  Returns [(x, info)] in input order."""
+    import concurrent.futures
+
+    import torch
+
+    main = torch.cuda.current_stream()
+    streams = [torch.cuda.Stream(device=b.device) for b in rhs]
+    for s in streams:
+        s.wait_stream(main)  # the right-hand sides were produced on the current stream
+
+    def run(k):
+        with torch.cuda.stream(streams[k]):
+            x, info = solvers[k].solve(rhs[k], rtol=rtol, max_it=max_it, restart=restart)
+        streams[k].synchronize()
+        return x, info
+
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(solvers)) as ex:
+        out = list(ex.map(run, range(len(solvers))))
+    for s in streams:
+        main.wait_stream(s)
+    return out

@@ -151,6 +151,38 @@ def test_full_block(sinker):
     assert rel(yu, ru) <= 1e-12 and rel(yp, rp) <= 1e-12
 
 
+@pytest.mark.parametrize("dims", [(12, 12, 12), (9, 6, 5)])
+def test_full_block_fp32(dims):
+    """FP32 full block (one fused pass, FP32-stored factors) against the FP64 oracle on a
+    sinker with viscosity contrast 1e6, MatMult semantics with free-slip walls; raw ADD mode
+    adds onto the output; repeated calls are bitwise identical."""
+    mx, my, mz = dims
+    say("case: FP32 full block %s contrast 1e6" % (dims,))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 2)
+    cen, r = M.sphere_inclusions(8, 0.1, 0.2, 6)
+    eta = M.viscosity_inclusions(M.qp_coords(mx, my, mz, c), cen, r, 1e6, 1.0, None)
+    mask = M.bc_mask(mx, my, mz, "freeslip")
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    ctx.set_viscosity(eta)
+    nvel, npres = 3 * M.nnodes(mx, my, mz), 4 * mx * my * mz
+    u = M.uniform(11, nvel, -1.0, 1.0)
+    p = M.uniform(12, npres, -1.0, 1.0)
+    ru, rp = O.matmult_a(mx, my, mz, c, eta, mask, u, p)
+    yu, yp = ctx.stokes(u.astype(np.float32), p.astype(np.float32), flags=_lib.MATMULT | _lib.FP32)
+    assert yu.dtype == np.float32
+    assert rel(yu.astype(np.float64), ru) <= 1e-5 and rel(yp.astype(np.float64), rp) <= 1e-5
+    yu2, yp2 = ctx.stokes(u.astype(np.float32), p.astype(np.float32), flags=_lib.MATMULT | _lib.FP32)
+    assert np.array_equal(yu, yu2) and np.array_equal(yp, yp2)
+    # raw mode adds the element sums (no BC steps) onto the output
+    au = M.uniform(13, nvel, -1.0, 1.0).astype(np.float32)
+    ap = M.uniform(14, npres, -1.0, 1.0).astype(np.float32)
+    zu, zp = ctx.stokes(u.astype(np.float32), p.astype(np.float32), flags=_lib.FP32)
+    wu, wp = ctx.stokes(u.astype(np.float32), p.astype(np.float32), au.copy(), ap.copy(), flags=_lib.ADD | _lib.FP32)
+    assert np.abs(wu - (zu + au)).max() <= 1e-5 * np.abs(zu).max()
+    assert np.abs(wp - (zp + ap)).max() <= 1e-5 * np.abs(zp).max()
+
+
 def test_mesh_box_with_ghost_layers():
     """A rank-local ghosted node box: elements placed at an offset inside a larger box give
     the same element sums on their nodes and leave the other nodes untouched."""
