@@ -221,6 +221,9 @@ def main():
     achieved = bytes_alg / (ms * 1e-3) / 1e9
     launches = 8
     extra = {}
+    ms_otf = time_applies(lambda: A.ctx.a11(u, y, _lib.ONTHEFLY), args.steps, args.warmup, stream)
+    extra["fp64_onthefly_ms"] = round(ms_otf, 4)
+    extra["fp64_onthefly_gdofs"] = round(P.nvel / (ms_otf * 1e-3) / 1e9, 3)
     if args.fp32:
         u32 = u.float()
         y32 = torch.empty_like(u32)

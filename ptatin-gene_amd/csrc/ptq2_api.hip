@@ -116,6 +116,18 @@ namespace {
 
 int build_fac32(ptq2_ctx *c) { return ptq2_internal_build_fac32(c); }
 
+// resident FP64 geometry of the stored-geometry A11 kernel (depends on the mesh only)
+int build_geo64(ptq2_ctx *c) {
+  if (c->geo64_valid) return PTQ2_OK;
+  if (!c->geo64) {
+    int rc = dev_alloc(c, (void **)&c->geo64, sizeof(double) * (size_t)c->nbricks * 243 * ptq2::BEL);
+    if (rc) return rc;
+  }
+  HIPCHK(c, ptq2::launch_build_geo64(c->box, c->coords, c->geo64, c->stream));
+  c->geo64_valid = true;
+  return PTQ2_OK;
+}
+
 int check_ready(ptq2_ctx *c, bool need_eta) {
   if (!c) return PTQ2_ERR_INVALID_ARG;
   if (!c->mesh_ready) return fail(c, PTQ2_ERR_NOT_SETUP, "MF operator not setup: no mesh");
@@ -189,7 +201,8 @@ int ptq2_destroy(ptq2_ctx **pc) {
   ptq2_ctx *c = *pc;
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
-  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm); dev_free(c->fac32); dev_free(c->facb32);
+  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
+  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64);
   dev_free(c->err); dev_free(c->su); dev_free(c->sy); dev_free(c->sp); dev_free(c->syp);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -242,9 +255,11 @@ int ptq2_setup_mesh_box(ptq2_ctx *c, int mx, int my, int mz, int NX, int NY, int
 
   // release a previous mesh
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm); dev_free(c->fac32); dev_free(c->facb32);
-  c->coords = nullptr; c->mask = nullptr; c->dir_idx = nullptr; c->eta_bm = nullptr; c->fac32 = nullptr; c->facb32 = nullptr;
-  c->mesh_ready = c->eta_ready = c->fac32_valid = c->facb32_valid = c->has_state = false;
+  dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
+  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64);
+  c->coords = nullptr; c->mask = nullptr; c->dir_idx = nullptr; c->eta_bm = nullptr;
+  c->fac32 = nullptr; c->facb32 = nullptr; c->geo64 = nullptr;
+  c->mesh_ready = c->eta_ready = c->fac32_valid = c->facb32_valid = c->geo64_valid = c->has_state = false;
   c->ndir = 0;
 
   Box b;
@@ -346,14 +361,17 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
   int rc = check_ready(c, true);
   if (rc) return rc;
   if (!u || !y) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
-  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
+  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY))
+    return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
   if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
   if ((rc = set_device(c))) return rc;
   const bool dev = flags & PTQ2_DEVICE, f32 = flags & PTQ2_FP32, mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
+  const bool otf = flags & PTQ2_ONTHEFLY;
   const uint8_t *mask = mm ? c->mask : nullptr;
   const int64_t n = c->nvel;
   const size_t es = f32 ? sizeof(float) : sizeof(double);
   if (f32 && (rc = build_fac32(c))) return rc;
+  if (!f32 && !otf && (rc = build_geo64(c))) return rc;
   const void *du = u;
   void *dy = y;
   if (!dev) {
@@ -369,7 +387,12 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
     HIPCHK(c, ptq2::launch_a11_fp32(c->box, (const float *)du, mask, c->fac32, (float *)dy, add, c->stream));
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, (const float *)du, (float *)dy, 0, c->stream));
   } else {
-    HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dy, add, c->stream));
+    if (otf)
+      HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dy, add,
+                                      c->stream));
+    else
+      HIPCHK(c, ptq2::launch_a11_fp64g(c->box, (const double *)du, mask, c->geo64, c->eta_bm, (double *)dy, add,
+                                       c->stream));
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, (const double *)du, (double *)dy, 0, c->stream));
   }
   if (!dev) {
@@ -466,13 +489,16 @@ int ptq2_stokes_apply(ptq2_ctx *c, const void *u, const void *p, void *yu, void 
   int rc = check_ready(c, true);
   if (rc) return rc;
   if (!u || !p || !yu || !yp) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
-  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
+  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY))
+    return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
   if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
   if ((rc = set_device(c))) return rc;
   const bool dev = flags & PTQ2_DEVICE, f32 = flags & PTQ2_FP32, mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
+  const bool otf = flags & PTQ2_ONTHEFLY;
   const uint8_t *mask = mm ? c->mask : nullptr;
   const size_t es = f32 ? sizeof(float) : sizeof(double);
   if (f32 && (rc = ptq2_internal_build_fac32(c, true))) return rc;
+  if (!f32 && !otf && (rc = build_geo64(c))) return rc;
   const void *du = u, *dp = p;
   void *dyu = yu, *dyp = yp;
   if (!dev) {
@@ -499,8 +525,12 @@ int ptq2_stokes_apply(ptq2_ctx *c, const void *u, const void *p, void *yu, void 
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, (const float *)du, (float *)dyu, 0, c->stream));
   } else {
     // momentum A11 u + A12 p, continuity A21 u (MFStokesWrapper_A_AVX in three passes)
-    HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dyu, add,
-                                    c->stream));
+    if (otf)
+      HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dyu, add,
+                                      c->stream));
+    else
+      HIPCHK(c, ptq2::launch_a11_fp64g(c->box, (const double *)du, mask, c->geo64, c->eta_bm, (double *)dyu, add,
+                                       c->stream));
     HIPCHK(c, ptq2::launch_a12(c->box, c->coords, (const double *)dp, (double *)dyu, true, c->stream));
     HIPCHK(c, ptq2::launch_a21(c->box, c->coords, (const double *)du, mask, (double *)dyp, add, c->stream));
     if (mm)

@@ -21,6 +21,8 @@ struct ptq2_ctx {
   double *eta_bm = nullptr;
   float *fac32 = nullptr;
   bool fac32_valid = false;
+  double *geo64 = nullptr;  // FP64 per-point geometry sqrt(w/det) adj(J) (mesh only)
+  bool geo64_valid = false;
   float *facb32 = nullptr;  // FP32 full-block factors (w det / eta scale, pressure basis)
   bool facb32_valid = false;
   bool has_state = false;

@@ -17,6 +17,9 @@ hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, co
                            hipStream_t s);
 hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, float *facb,
                               int *err, hipStream_t s);
+hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,
+                            double *y, bool add, hipStream_t s);
+hipError_t launch_build_geo64(const Box &b, const double *coords, double *geo, hipStream_t s);
 hipError_t launch_stokes_fp32(const Box &b, const float *u, const float *p, const uint8_t *mask, const float *fac,
                               const float *facb, float *yu, float *yp, bool add, hipStream_t s);
 hipError_t launch_eta_to_bm(const Box &b, const double *eta, double *eta_bm, int *err, hipStream_t s);

@@ -241,6 +241,133 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   store_tile(b, br, t, Us, y, add_mode != 0, lane);
 }
 
+// ------------------------------------------------------ A11 FP64, stored geometry
+// The same operator with the per-point geometry G = sqrt(w/det) adj(J) resident in HBM
+// (FP64, built once per mesh by k_build_geo64) and eta applied at run time:
+// M = G grad u, S = eta (M + M^T), dv = G^T S equals w det J^-T (2 eta eps(u)).  Trades
+// 1944 B/element of reads for the coordinate tile and the Jacobian work of k_a11_fp64.
+//   geo [brick][q][9][lane] doubles
+template <int qa>
+__device__ __forceinline__ void a11_qp64g(const double ur0[3][3], const double ur1[3][3], const double ur2[3][3],
+                                          const double G9[9], double eta, double P0[3][3], double P1[3][3],
+                                          double P2[3][3]) {
+  double g[3][3], dv[3][3];
+  qp_grad<qa>(ur0, ur1, ur2, g);
+  double M[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) M[k][l] = g[0][l] * G9[3 * k] + g[1][l] * G9[3 * k + 1] + g[2][l] * G9[3 * k + 2];
+  double S[3][3];
+  const double e2 = eta + eta;
+  S[0][0] = e2 * M[0][0];
+  S[1][1] = e2 * M[1][1];
+  S[2][2] = e2 * M[2][2];
+  S[0][1] = S[1][0] = eta * (M[0][1] + M[1][0]);
+  S[0][2] = S[2][0] = eta * (M[0][2] + M[2][0]);
+  S[1][2] = S[2][1] = eta * (M[1][2] + M[2][1]);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * G9[k] + S[1][l] * G9[3 + k] + S[2][l] * G9[6 + k];
+  acc_row<qa>(dv, P0, P1, P2);
+}
+
+__global__ __launch_bounds__(BEL) void k_a11_fp64g(Box b, int color, const double *__restrict__ u,
+                                                   const uint8_t *__restrict__ mask, const double *__restrict__ geo,
+                                                   const double *__restrict__ eta_bm, double *__restrict__ y,
+                                                   int add_mode) {
+  __shared__ double Us[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_of(b, color, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  const double *gbase = geo + (int64_t)br.lin * 243 * BEL + lane;
+  const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
+  load_tile(b, t, br, u, mask, Us, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  double Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
+  if (active) {
+    double ca[9], cb9[9];
+    ld_col(Us, base, 0, ca);
+#pragma nounroll
+    for (int row = 0; row < 9; row++) {
+      const int qb = row % 3, qc = row / 3;
+      const Coef1 cb = coef1(qb), cc = coef1(qc);
+      double G[27], e3[3];
+#pragma unroll
+      for (int n = 0; n < 27; n++) G[n] = gbase[(int64_t)(27 * row + n) * BEL];
+#pragma unroll
+      for (int q = 0; q < 3; q++) e3[q] = ep[(3 * row + q) * BEL];
+      double ur0[3][3], ur1[3][3], ur2[3][3];
+      row_data_pipe(Us, Us, base, cb, cc, ca, cb9, ur0, ur1, ur2);
+#pragma unroll
+      for (int n = 0; n < 9; n++) ca[n] = cb9[n];
+      double P0[3][3], P1[3][3], P2[3][3];
+      zero33(P0); zero33(P1); zero33(P2);
+      a11_qp64g<0>(ur0, ur1, ur2, G, e3[0], P0, P1, P2);
+      a11_qp64g<1>(ur0, ur1, ur2, G + 9, e3[1], P0, P1, P2);
+      a11_qp64g<2>(ur0, ur1, ur2, G + 18, e3[2], P0, P1, P2);
+      row_fence();
+      acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
+      row_fence();
+    }
+  }
+  __syncthreads();
+  zero_tile(Us, lane);
+  __syncthreads();
+  if (active) emit_element(Us, base, Y);
+  __syncthreads();
+  store_tile(b, br, t, Us, y, add_mode != 0, lane);
+}
+
+// G = sqrt(w/det) adj(J) per point (FP64), for k_a11_fp64g
+template <int qb, int qc>
+__device__ __forceinline__ void geo_row(const double *Xs, int base, double *__restrict__ gp) {
+  double xr0[3][3], xr1[3][3], xr2[3][3];
+  row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+#pragma unroll
+  for (int qa = 0; qa < 3; qa++) {
+    double J[3][3], A[3][3];
+    switch (qa) {
+      case 0: qp_grad<0>(xr0, xr1, xr2, J); break;
+      case 1: qp_grad<1>(xr0, xr1, xr2, J); break;
+      default: qp_grad<2>(xr0, xr1, xr2, J); break;
+    }
+    const double det = adjugate(J, A);
+    const int q = qa + 3 * qb + 9 * qc;
+    const double w = gw1(qa) * gw1(qb) * gw1(qc);
+    const double sc = det > 0.0 ? sqrt(w / det) : 0.0;
+#pragma unroll
+    for (int m = 0; m < 9; m++) gp[(q * 9 + m) * BEL] = sc * A[m / 3][m % 3];
+  }
+  row_fence();
+}
+
+__global__ __launch_bounds__(BEL) void k_build_geo64(Box b, const double *__restrict__ coords, double *__restrict__ geo) {
+  __shared__ double Xs[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_lin(b, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, br, coords, (const uint8_t *)nullptr, Xs, lane);
+  __syncthreads();
+  if (!lane_active(t, lane)) return;
+  const int base = lane_base(lane);
+  double *gp = geo + (int64_t)br.lin * 243 * BEL + lane;
+  geo_row<0, 0>(Xs, base, gp); geo_row<1, 0>(Xs, base, gp); geo_row<2, 0>(Xs, base, gp);
+  geo_row<0, 1>(Xs, base, gp); geo_row<1, 1>(Xs, base, gp); geo_row<2, 1>(Xs, base, gp);
+  geo_row<0, 2>(Xs, base, gp); geo_row<1, 2>(Xs, base, gp); geo_row<2, 2>(Xs, base, gp);
+}
+
 // ------------------------------------------------- FP32 full block [A11 A12; A21 0]
 // One pass: the A11 point action of k_a11_fp32 plus, per point, p_q = pe . N^p and
 //   dv[k][l] -= p_q w adj(J)[l][k]            (A12, QuadratureAction_A12_AVX)
@@ -770,6 +897,21 @@ hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, co
 hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, float *facb,
                               int *err, hipStream_t s) {
   hipLaunchKernelGGL(k_build_fac32, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, eta_bm, fac, facb, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,
+                            double *y, bool add, hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_a11_fp64g, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, geo, eta_bm, y, add ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_build_geo64(const Box &b, const double *coords, double *geo, hipStream_t s) {
+  hipLaunchKernelGGL(k_build_geo64, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, geo);
   return hipGetLastError();
 }
 

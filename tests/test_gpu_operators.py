@@ -71,6 +71,30 @@ def test_a11_fp64_sinker_matmult(sinker):
     assert rel(y32, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("dims", [(20, 20, 20), (9, 4, 5), (1, 1, 3)])
+def test_a11_fp64_geometry_variants(dims):
+    """The resident-geometry kernel (default) and the on-the-fly kernel (PTQ2_ONTHEFLY)
+    against the oracle, MatMult and raw ADD, including partial bricks."""
+    mx, my, mz = dims
+    say("case: A11 FP64 stored vs on-the-fly geometry %s" % (dims,))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 3)
+    c = M.affine(c, M.SHEAR, [0.0, 0.0, 0.0])
+    cen, r = M.sphere_inclusions(6, 0.1, 0.2, 7)
+    eta = M.viscosity_inclusions(M.qp_coords(mx, my, mz, c), cen, r, 1e3, 1.0, 8)
+    mask = M.bc_mask(mx, my, mz, "freeslip_freesurface")
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    ctx.set_viscosity(eta)
+    n = 3 * M.nnodes(mx, my, mz)
+    u = M.uniform(15, n, -1.0, 1.0)
+    y0 = M.uniform(16, n, -1.0, 1.0)
+    ref = O.matmult_a11(mx, my, mz, c, eta, mask, u)
+    raw = y0 + O.a11_tensor(mx, my, mz, c, eta, u, threads=8)
+    for fl in (0, _lib.ONTHEFLY):
+        assert rel(ctx.a11(u, flags=_lib.MATMULT | fl), ref) <= 1e-12
+        assert rel(ctx.a11(u, y0.copy(), flags=_lib.ADD | fl), raw) <= 1e-12
+
+
 def test_a11_raw_add_mode(sinker):
     P = sinker
     m = P.mx
