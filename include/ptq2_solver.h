@@ -61,7 +61,10 @@ int ptq2_mg_vcycle(ptq2_mg *mg, const double *r, double *z);
 
 /* FGMRES(restart) with the V-cycle as right preconditioner on device FP64 vectors:
    solves MatMult_MFStokes_A11 x = b from x = 0 until ||b - A x|| <= rtol ||b||.
-   Returns the iteration count and the final true residual norm. */
+   1 <= restart <= 256; the solver keeps 2*restart+1 device vectors of the fine size
+   (high-contrast problems with many stiff inclusions need long restarts: their
+   near-rigid modes form a cluster of small eigenvalues).  Returns the iteration count
+   and the final true residual norm. */
 int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_it, int restart,
                   int *iterations, double *final_rnorm, double *rnorm0);
 

@@ -112,6 +112,13 @@ int ptq2_internal_build_fac32(ptq2_ctx *c, bool block) {
   return PTQ2_OK;
 }
 
+void *ptq2_internal_tilebuf(ptq2_ctx *c, int *rc) {
+  *rc = PTQ2_OK;
+  if (c->nbricks >= 8 * (int64_t)c->simd_slots) return nullptr;
+  if (!c->tilebuf) *rc = dev_alloc(c, &c->tilebuf, sizeof(double) * (size_t)c->nbricks * ptq2::TILEVALS);
+  return c->tilebuf;
+}
+
 namespace {
 
 int build_fac32(ptq2_ctx *c) { return ptq2_internal_build_fac32(c); }
@@ -186,6 +193,10 @@ int ptq2_create(ptq2_ctx **out, int device) {
     return PTQ2_ERR_DEVICE;
   }
   c->stream = c->own_stream;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+    c->simd_slots = 4 * cus;
+  (void)hipGetLastError();
   if (hipMalloc((void **)&c->err, sizeof(int)) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipStreamDestroy(c->own_stream);
@@ -202,7 +213,7 @@ int ptq2_destroy(ptq2_ctx **pc) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
-  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64);
+  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->tilebuf);
   dev_free(c->err); dev_free(c->su); dev_free(c->sy); dev_free(c->sp); dev_free(c->syp);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -256,9 +267,9 @@ int ptq2_setup_mesh_box(ptq2_ctx *c, int mx, int my, int mz, int NX, int NY, int
   // release a previous mesh
   HIPCHK(c, hipStreamSynchronize(c->stream));
   dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
-  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64);
+  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->tilebuf);
   c->coords = nullptr; c->mask = nullptr; c->dir_idx = nullptr; c->eta_bm = nullptr;
-  c->fac32 = nullptr; c->facb32 = nullptr; c->geo64 = nullptr;
+  c->fac32 = nullptr; c->facb32 = nullptr; c->geo64 = nullptr; c->tilebuf = nullptr;
   c->mesh_ready = c->eta_ready = c->fac32_valid = c->facb32_valid = c->geo64_valid = c->has_state = false;
   c->ndir = 0;
 
@@ -383,16 +394,19 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
     du = c->su;
     dy = c->sy;
   }
+  void *tb = ptq2_internal_tilebuf(c, &rc);
+  if (rc) return rc;
   if (f32) {
-    HIPCHK(c, ptq2::launch_a11_fp32(c->box, (const float *)du, mask, c->fac32, (float *)dy, add, c->stream));
+    HIPCHK(c, ptq2::launch_a11_fp32(c->box, (const float *)du, mask, c->fac32, (float *)dy, add, c->stream,
+                                    (float *)tb));
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, (const float *)du, (float *)dy, 0, c->stream));
   } else {
     if (otf)
       HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dy, add,
-                                      c->stream));
+                                      c->stream, (double *)tb));
     else
       HIPCHK(c, ptq2::launch_a11_fp64g(c->box, (const double *)du, mask, c->geo64, c->eta_bm, (double *)dy, add,
-                                       c->stream));
+                                       c->stream, (double *)tb));
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, (const double *)du, (double *)dy, 0, c->stream));
   }
   if (!dev) {

@@ -28,6 +28,12 @@ struct ptq2_ctx {
   bool has_state = false;
   uint64_t state = 0;
   int *err = nullptr;
+  // output path of the A11 applies: meshes whose bricks cannot fill 8 coloured launches
+  // (fewer than 8 x the device's wave slots) run all bricks in one launch and sum the
+  // tiles with a gather pass (ptq2::launch_a11_*'s tilebuf); tilebuf holds TILEVALS
+  // doubles per brick
+  int simd_slots = 1024;
+  void *tilebuf = nullptr;
   // staging for host-pointer calls
   void *su = nullptr, *sy = nullptr;
   size_t su_bytes = 0, sy_bytes = 0;
@@ -40,3 +46,6 @@ struct ptq2_ctx {
 // (re)build the FP32 geometric factors of the smoother operator if stale; with block,
 // also those of the FP32 full block
 int ptq2_internal_build_fac32(ptq2_ctx *c, bool block = false);
+// the tile buffer of the uncoloured output path if this mesh uses it (allocated on first
+// use), else nullptr; *rc receives an allocation error
+void *ptq2_internal_tilebuf(ptq2_ctx *c, int *rc);

@@ -261,6 +261,20 @@ __device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, cons
   }
 }
 
+// Uncoloured output: the whole LDS output tile of brick br to its slot of a tile
+// buffer, dense [c][z][y][x] with 9 nodes per edge (k_gather sums the slots).
+constexpr int TILEVALS = 3 * TN * TN * TN;
+template <typename T>
+__device__ __forceinline__ void store_tile_buf(const BrickId &br, const T *__restrict__ s, T *__restrict__ buf,
+                                               int lane) {
+  T *o = buf + (int64_t)br.lin * TILEVALS;
+  for (int f = lane; f < TILEVALS; f += BEL) {
+    const int c = f / (TN * TN * TN), r = f - c * (TN * TN * TN);
+    const int z = r / (TN * TN), y = (r / TN) % TN, x = r % TN;
+    o[f] = s[c * TCS + z * TPS + y * TRS + tx(x)];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void zero_tile(T *__restrict__ s, int lane) {
   for (int f = lane; f < 3 * TCS; f += BEL) s[f] = T(0);
@@ -281,6 +295,11 @@ struct LaneElem {
 __host__ __device__ __forceinline__ LaneElem lane_elem(int lane) {
   const int g = lane >> 4;
   return LaneElem{lane & (BE - 1), (lane >> 2) & (BE - 1), ((g & 1) << 1) | (g >> 1)};
+}
+// inverse of lane_elem: the lane of element (lx,ly,lz) of a brick (the z order swap is
+// its own inverse)
+__host__ __device__ __forceinline__ int elem_lane(int lx, int ly, int lz) {
+  return lx + BE * ly + BE * BE * (((lz & 1) << 1) | (lz >> 1));
 }
 __device__ __forceinline__ int lane_base(int lane) {
   const LaneElem e = lane_elem(lane);

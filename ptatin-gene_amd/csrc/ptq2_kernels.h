@@ -11,14 +11,16 @@ constexpr int ERRBIT_MESH = 1;
 constexpr int ERRBIT_VISCOSITY = 2;
 constexpr int ERRBIT_NONFINITE = 4;
 
+// A11 launchers: with tilebuf (TILEVALS values per brick) all bricks run in one launch
+// and k_gather sums the tiles; without it, 8 coloured launches store/add directly.
 hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
-                           const double *eta_bm, double *y, bool add, hipStream_t s);
+                           const double *eta_bm, double *y, bool add, hipStream_t s, double *tilebuf = nullptr);
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
-                           hipStream_t s);
+                           hipStream_t s, float *tilebuf = nullptr);
 hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, float *facb,
                               int *err, hipStream_t s);
 hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,
-                            double *y, bool add, hipStream_t s);
+                            double *y, bool add, hipStream_t s, double *tilebuf = nullptr);
 hipError_t launch_build_geo64(const Box &b, const double *coords, double *geo, hipStream_t s);
 hipError_t launch_stokes_fp32(const Box &b, const float *u, const float *p, const uint8_t *mask, const float *fac,
                               const float *facb, float *yu, float *yp, bool add, hipStream_t s);

@@ -7,6 +7,8 @@
 #include "ptq2_device.h"
 #include "ptq2_kernels.h"
 
+#include <algorithm>
+
 namespace ptq2 {
 
 // ----------------------------------------------------------------- A11 FP64
@@ -77,11 +79,11 @@ __device__ __forceinline__ void load3(const double *__restrict__ ep, int row, do
 __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double *__restrict__ coords,
                                                   const double *__restrict__ u, const uint8_t *__restrict__ mask,
                                                   const double *__restrict__ eta_bm, double *__restrict__ y,
-                                                  int add_mode) {
+                                                  int add_mode, double *__restrict__ tilebuf) {
   __shared__ double Xs[3 * TCS];
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
-  const BrickId br = brick_of(b, color, blockIdx.x);
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_lin(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
   double en[3];
@@ -115,7 +117,10 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
   __syncthreads();
   if (active) emit_element(Xs, base, Y);
   __syncthreads();
-  store_tile(b, br, t, Xs, y, add_mode != 0, lane);
+  if (color >= 0)
+    store_tile(b, br, t, Xs, y, add_mode != 0, lane);
+  else
+    store_tile_buf(br, Xs, tilebuf, lane);
 }
 
 // ----------------------------------------------------------------- A11 FP32
@@ -176,10 +181,10 @@ __device__ __forceinline__ void slab_z32(const float *Us, int base, const Coef1 
 
 __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
                                                   const uint8_t *__restrict__ mask, const float *__restrict__ fac,
-                                                  float *__restrict__ y, int add_mode) {
+                                                  float *__restrict__ y, int add_mode, float *__restrict__ tilebuf) {
   __shared__ float Us[3 * TCS];
   const int lane = threadIdx.x;
-  const BrickId br = brick_of(b, color, blockIdx.x);
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_lin(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
   load_tile(b, t, br, u, mask, Us, lane);
@@ -238,7 +243,10 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   __syncthreads();
   if (active) emit_element(Us, base, Y);
   __syncthreads();
-  store_tile(b, br, t, Us, y, add_mode != 0, lane);
+  if (color >= 0)
+    store_tile(b, br, t, Us, y, add_mode != 0, lane);
+  else
+    store_tile_buf(br, Us, tilebuf, lane);
 }
 
 // ------------------------------------------------------ A11 FP64, stored geometry
@@ -276,10 +284,10 @@ __device__ __forceinline__ void a11_qp64g(const double ur0[3][3], const double u
 __global__ __launch_bounds__(BEL) void k_a11_fp64g(Box b, int color, const double *__restrict__ u,
                                                    const uint8_t *__restrict__ mask, const double *__restrict__ geo,
                                                    const double *__restrict__ eta_bm, double *__restrict__ y,
-                                                   int add_mode) {
+                                                   int add_mode, double *__restrict__ tilebuf) {
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
-  const BrickId br = brick_of(b, color, blockIdx.x);
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_lin(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const double *gbase = geo + (int64_t)br.lin * 243 * BEL + lane;
   const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
@@ -327,7 +335,10 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64g(Box b, int color, const doubl
   __syncthreads();
   if (active) emit_element(Us, base, Y);
   __syncthreads();
-  store_tile(b, br, t, Us, y, add_mode != 0, lane);
+  if (color >= 0)
+    store_tile(b, br, t, Us, y, add_mode != 0, lane);
+  else
+    store_tile_buf(br, Us, tilebuf, lane);
 }
 
 // G = sqrt(w/det) adj(J) per point (FP64), for k_a11_fp64g
@@ -874,22 +885,71 @@ static inline int nbricks_color(const Box &b, int color) {
 }
 static inline int nbricks(const Box &b) { return b.nbx * b.nby * b.nbz; }
 
+// Sum of the tile-buffer slots of every covered node, bricks in increasing order
+// (deterministic): the uncoloured output path of the A11 kernels.
+template <typename T>
+__global__ void k_gather(Box b, const T *__restrict__ buf, T *__restrict__ y, int add_mode) {
+  const int64_t nx = 2 * (int64_t)b.mx + 1, ny = 2 * (int64_t)b.my + 1, nz = 2 * (int64_t)b.mz + 1;
+  const int64_t nn = nx * ny * nz;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nn; t += (int64_t)gridDim.x * blockDim.x) {
+    const int li = (int)(t % nx), lj = (int)((t / nx) % ny), lk = (int)(t / (nx * ny));
+    // (brick, local node) candidates per axis: the brick holding the node, and the previous
+    // brick when the node is on its upper face
+    int bxs[2], ixs[2], nxc = 0, bys[2], iys[2], nyc = 0, bzs[2], izs[2], nzc = 0;
+    if (li % (2 * BE) == 0 && li > 0) { bxs[nxc] = li / (2 * BE) - 1; ixs[nxc++] = 2 * BE; }
+    if (li / (2 * BE) < b.nbx) { bxs[nxc] = li / (2 * BE); ixs[nxc++] = li % (2 * BE); }
+    if (lj % (2 * BE) == 0 && lj > 0) { bys[nyc] = lj / (2 * BE) - 1; iys[nyc++] = 2 * BE; }
+    if (lj / (2 * BE) < b.nby) { bys[nyc] = lj / (2 * BE); iys[nyc++] = lj % (2 * BE); }
+    if (lk % (2 * BE) == 0 && lk > 0) { bzs[nzc] = lk / (2 * BE) - 1; izs[nzc++] = 2 * BE; }
+    if (lk / (2 * BE) < b.nbz) { bzs[nzc] = lk / (2 * BE); izs[nzc++] = lk % (2 * BE); }
+    T acc[3] = {T(0), T(0), T(0)};
+    for (int cz = 0; cz < nzc; cz++)
+      for (int cy = 0; cy < nyc; cy++)
+        for (int cx = 0; cx < nxc; cx++) {
+          const int64_t lin = bxs[cx] + (int64_t)b.nbx * (bys[cy] + (int64_t)b.nby * bzs[cz]);
+          const T *o = buf + lin * TILEVALS + (izs[cz] * TN + iys[cy]) * TN + ixs[cx];
+#pragma unroll
+          for (int c = 0; c < 3; c++) acc[c] += o[c * TN * TN * TN];
+        }
+    const int64_t gi = 3 * ((int64_t)(b.i0 + li) + (int64_t)b.NX * ((int64_t)(b.j0 + lj) + (int64_t)b.NY * (b.k0 + lk)));
+#pragma unroll
+    for (int c = 0; c < 3; c++) y[gi + c] = add_mode ? y[gi + c] + acc[c] : acc[c];
+  }
+}
+
+template <typename T>
+hipError_t launch_gather(const Box &b, const T *buf, T *y, bool add, hipStream_t s) {
+  const int64_t nn = (2 * (int64_t)b.mx + 1) * (2 * (int64_t)b.my + 1) * (2 * (int64_t)b.mz + 1);
+  const int blocks = (int)std::min<int64_t>((nn + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_gather<T>, dim3(blocks), dim3(256), 0, s, b, buf, y, add ? 1 : 0);
+  return hipGetLastError();
+}
+
 hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
-                           const double *eta_bm, double *y, bool add, hipStream_t s) {
+                           const double *eta_bm, double *y, bool add, hipStream_t s, double *tilebuf) {
+  if (tilebuf) {
+    hipLaunchKernelGGL(k_a11_fp64, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm, y, 0, tilebuf);
+    return launch_gather(b, (const double *)tilebuf, y, add, s);
+  }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
-    hipLaunchKernelGGL(k_a11_fp64, dim3(nb), dim3(BEL), 0, s, b, c, coords, u, mask, eta_bm, y, add ? 1 : 0);
+    hipLaunchKernelGGL(k_a11_fp64, dim3(nb), dim3(BEL), 0, s, b, c, coords, u, mask, eta_bm, y, add ? 1 : 0,
+                       (double *)nullptr);
   }
   return hipGetLastError();
 }
 
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
-                           hipStream_t s) {
+                           hipStream_t s, float *tilebuf) {
+  if (tilebuf) {
+    hipLaunchKernelGGL(k_a11_fp32, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, fac, y, 0, tilebuf);
+    return launch_gather(b, (const float *)tilebuf, y, add, s);
+  }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
-    hipLaunchKernelGGL(k_a11_fp32, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, fac, y, add ? 1 : 0);
+    hipLaunchKernelGGL(k_a11_fp32, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, fac, y, add ? 1 : 0, (float *)nullptr);
   }
   return hipGetLastError();
 }
@@ -901,11 +961,16 @@ hipError_t launch_build_fac32(const Box &b, const double *coords, const double *
 }
 
 hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,
-                            double *y, bool add, hipStream_t s) {
+                            double *y, bool add, hipStream_t s, double *tilebuf) {
+  if (tilebuf) {
+    hipLaunchKernelGGL(k_a11_fp64g, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, geo, eta_bm, y, 0, tilebuf);
+    return launch_gather(b, (const double *)tilebuf, y, add, s);
+  }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
-    hipLaunchKernelGGL(k_a11_fp64g, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, geo, eta_bm, y, add ? 1 : 0);
+    hipLaunchKernelGGL(k_a11_fp64g, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, geo, eta_bm, y, add ? 1 : 0,
+                       (double *)nullptr);
   }
   return hipGetLastError();
 }

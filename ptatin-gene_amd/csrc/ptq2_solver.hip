@@ -26,6 +26,9 @@ using ptq2::Box;
 namespace {
 
 constexpr int TPB = 256;
+// largest FGMRES restart length (Krylov pointer tables and reduction buffers are sized
+// for it; slot MAXRESTART of the Z table is a scratch pointer for norms)
+constexpr int MAXRESTART = 256;
 
 inline unsigned nblk(int64_t n, int64_t cap = 16384) {
   int64_t b = (n + TPB - 1) / TPB;
@@ -170,7 +173,7 @@ __global__ void k_eta_coarsen(Box f, int mxc, int myc, int mzc, const double *__
           if ((qa == 0 && a == 1) || (qa == 2 && a == 0)) continue;
           int ex = 2 * cx + a, ey = 2 * cy + bb, ez = 2 * cz + c;
           int64_t lin = (ex >> 2) + (int64_t)f.nbx * ((ey >> 2) + (int64_t)f.nby * (ez >> 2));
-          int lane = (ex & 3) + 4 * (ey & 3) + 16 * (ez & 3);
+          int lane = ptq2::elem_lane(ex & 3, ey & 3, ez & 3);
           const double *src = eta_bm_f + lin * 27 * ptq2::BEL + lane;
           for (int qq = 0; qq < 27; qq++) s += src[qq * ptq2::BEL];
           cnt += 27;
@@ -181,22 +184,33 @@ __global__ void k_eta_coarsen(Box f, int mxc, int myc, int mzc, const double *__
   }
 }
 
-// partial dot products of w with nv vectors: part[v * nb + block]
+// partial dot products of w with nv vectors: part[v * nb + block].  The vectors are taken
+// MD at a time, so w is read once per group instead of once per vector.
+constexpr int MD = 8;
 __global__ void k_mdot(int64_t n, const double *const *__restrict__ V, int nv, const double *__restrict__ w,
                        double *__restrict__ part) {
-  __shared__ double red[TPB / 64];
-  for (int v = 0; v < nv; v++) {
-    const double *a = V[v];
-    double s = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-      s += a[i] * w[i];
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x / 64] = s;
+  __shared__ double red[MD][TPB / 64];
+  for (int v0 = 0; v0 < nv; v0 += MD) {
+    const int nm = min(MD, nv - v0);
+    double s[MD];
+#pragma unroll
+    for (int m = 0; m < MD; m++) s[m] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+      const double wi = w[i];
+#pragma unroll
+      for (int m = 0; m < MD; m++)
+        if (m < nm) s[m] += V[v0 + m][i] * wi;
+    }
+#pragma unroll
+    for (int m = 0; m < MD; m++) {
+      for (int off = 32; off > 0; off >>= 1) s[m] += __shfl_down(s[m], off, 64);
+      if ((threadIdx.x & 63) == 0) red[m][threadIdx.x / 64] = s[m];
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < nm) {
       double t = 0.0;
-      for (int k = 0; k < TPB / 64; k++) t += red[k];
-      part[(int64_t)v * gridDim.x + blockIdx.x] = t;
+      for (int k = 0; k < TPB / 64; k++) t += red[threadIdx.x][k];
+      part[(int64_t)(v0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
     }
     __syncthreads();
   }
@@ -339,14 +353,21 @@ int level_matmult(ptq2_mg *mg, int l, const T *x, T *y);
 template <>
 int level_matmult<double>(ptq2_mg *mg, int l, const double *x, double *y) {
   ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream));
+  // the level's stored geometry and output path were prepared at set-up (no allocation
+  // here: this runs under graph capture)
+  if (c->geo64_valid)
+    MGCHK(ptq2::launch_a11_fp64g(c->box, x, c->mask, c->geo64, c->eta_bm, y, false, mg->stream,
+                                 (double *)c->tilebuf));
+  else
+    MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream,
+                                (double *)c->tilebuf));
   MGCHK(ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
   return PTQ2_OK;
 }
 template <>
 int level_matmult<float>(ptq2_mg *mg, int l, const float *x, float *y) {
   ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream));
+  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->tilebuf));
   MGCHK(ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
   return PTQ2_OK;
 }
@@ -448,7 +469,7 @@ int mdot(ptq2_mg *mg, const double **Vdev, int nv, const double *w, int64_t n, d
 
 int norm2(ptq2_mg *mg, const double *x, int64_t n, double *out) {
   // a pointer table entry pointing at x itself, stored after the Krylov tables
-  const double **slot = mg->Zptr + 64;
+  const double **slot = mg->Zptr + MAXRESTART;
   MGCHK(hipMemcpyAsync((void *)slot, &x, sizeof(double *), hipMemcpyHostToDevice, mg->stream));
   double d;
   MGRC(mdot(mg, slot, 1, x, n, &d));
@@ -456,7 +477,6 @@ int norm2(ptq2_mg *mg, const double *x, int64_t n, double *out) {
   return PTQ2_OK;
 }
 
-// dense Cholesky inverse of an SPD matrix (row-major, n x n), in place
 // ---------------------------------------------------------------- coarse inverse
 // Dense inverse of the coarsest operator by the block sweep (Gauss-Jordan elimination
 // without pivoting, valid for SPD matrices): sweeping diagonal block k with D = A_kk^-1
@@ -548,8 +568,9 @@ __global__ __launch_bounds__(256) void k_sweep_panel(int np, int k, const double
 __global__ __launch_bounds__(256) void k_sweep_update(int np, int k, double *__restrict__ A,
                                                       const double *__restrict__ P, const double *__restrict__ C,
                                                       const double *__restrict__ Dm) {
-  __shared__ double Ps[SB][SB + 1];
-  __shared__ double Cs[SB][SB + 1];
+  constexpr int KC = SB / 2;  // the rank-SB update in two K chunks keeps LDS at 33 KB
+  __shared__ double Ps[SB][KC + 1];
+  __shared__ double Cs[SB][KC + 1];
   const int bi = blockIdx.y, bj = blockIdx.x;
   const int t = threadIdx.x;
   const int64_t i0 = (int64_t)bi * SB, j0 = (int64_t)bj * SB;
@@ -567,11 +588,6 @@ __global__ __launch_bounds__(256) void k_sweep_update(int np, int k, double *__r
     }
     return;
   }
-  for (int e = t; e < SB * SB; e += 256) {
-    Ps[e / SB][e % SB] = P[(i0 + e / SB) * SB + e % SB];
-    Cs[e / SB][e % SB] = C[(j0 + e / SB) * SB + e % SB];
-  }
-  __syncthreads();
   // thread owns a 4x4 sub-tile: rows 4*(t/16).., cols 4*(t%16)..
   const int ra = 4 * (t / 16), ca = 4 * (t % 16);
   double acc[4][4];
@@ -579,16 +595,24 @@ __global__ __launch_bounds__(256) void k_sweep_update(int np, int k, double *__r
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int c = 0; c < 4; c++) acc[a][c] = A[(i0 + ra + a) * np + j0 + ca + c];
-  for (int q = 0; q < SB; q++) {
-    double pa[4], cc[4];
+  for (int q0 = 0; q0 < SB; q0 += KC) {
+    __syncthreads();
+    for (int e = t; e < SB * KC; e += 256) {
+      Ps[e / KC][e % KC] = P[(i0 + e / KC) * SB + q0 + e % KC];
+      Cs[e / KC][e % KC] = C[(j0 + e / KC) * SB + q0 + e % KC];
+    }
+    __syncthreads();
+    for (int q = 0; q < KC; q++) {
+      double pa[4], cc[4];
 #pragma unroll
-    for (int a = 0; a < 4; a++) pa[a] = Ps[ra + a][q];
+      for (int a = 0; a < 4; a++) pa[a] = Ps[ra + a][q];
 #pragma unroll
-    for (int c = 0; c < 4; c++) cc[c] = Cs[ca + c][q];
+      for (int c = 0; c < 4; c++) cc[c] = Cs[ca + c][q];
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+      for (int a = 0; a < 4; a++)
 #pragma unroll
-      for (int c = 0; c < 4; c++) acc[a][c] = fma(-pa[a], cc[c], acc[a][c]);
+        for (int c = 0; c < 4; c++) acc[a][c] = fma(-pa[a], cc[c], acc[a][c]);
+    }
   }
 #pragma unroll
   for (int a = 0; a < 4; a++)
@@ -718,10 +742,10 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   if (mg->n.back() > o.coarse_max_dofs) return mgfail(mg, PTQ2_ERR_SIZE, "coarse level too large for the dense solve");
 
   // ---- small reduction buffers (also used by the solve)
-  MGRC(mg_alloc(mg, (void **)&mg->part, sizeof(double) * 64 * 1024));
-  MGRC(mg_alloc(mg, (void **)&mg->hdev, sizeof(double) * 64));
-  MGRC(mg_alloc(mg, (void **)&mg->Vptr, sizeof(double *) * 64));
-  MGRC(mg_alloc(mg, (void **)&mg->Zptr, sizeof(double *) * 65));
+  MGRC(mg_alloc(mg, (void **)&mg->part, sizeof(double) * (MAXRESTART + 1) * 1024));
+  MGRC(mg_alloc(mg, (void **)&mg->hdev, sizeof(double) * (MAXRESTART + 1)));
+  MGRC(mg_alloc(mg, (void **)&mg->Vptr, sizeof(double *) * (MAXRESTART + 1)));
+  MGRC(mg_alloc(mg, (void **)&mg->Zptr, sizeof(double *) * (MAXRESTART + 1)));
 
   // ---- level vectors, Jacobi diagonals, eigen-estimates
   const size_t es = mg->f32 ? sizeof(float) : sizeof(double);
@@ -814,6 +838,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     (void)hipFree(e);
     MGRC(rc);
     hipLaunchKernelGGL(k_symmetrise, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, np, mg->ainv);
+    MGCHK(hipGetLastError());
     MGRC(mg_alloc(mg, (void **)&P, sizeof(double) * (size_t)np * SB));
     MGRC(mg_alloc(mg, (void **)&C, sizeof(double) * (size_t)np * SB));
     MGRC(mg_alloc(mg, (void **)&Dm, sizeof(double) * SB * SB));
@@ -825,6 +850,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
                          (const double *)Dm, P, C);
       hipLaunchKernelGGL(k_sweep_update, dim3(nbk, nbk), dim3(256), 0, mg->stream, np, k, mg->ainv,
                          (const double *)P, (const double *)C, (const double *)Dm);
+      MGCHK(hipGetLastError());
     }
     hipLaunchKernelGGL(k_negate, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, nn, mg->ainv);
     int herr = 0;
@@ -890,7 +916,7 @@ int ptq2_mg_vcycle(ptq2_mg *mg, const double *r, double *z) {
 
 int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_it, int restart, int *iters,
                   double *final_rnorm, double *rnorm0) {
-  if (!mg || !b || !x || restart < 1 || restart > 60 || max_it < 1) return PTQ2_ERR_INVALID_ARG;
+  if (!mg || !b || !x || restart < 1 || restart > MAXRESTART || max_it < 1) return PTQ2_ERR_INVALID_ARG;
   const int64_t n = mg->n[0];
   ptq2_ctx *fc = mg->ctx[0];
   // workspace
