@@ -77,7 +77,7 @@ def cpu_baseline(seconds=12.0):
             "sample": "%d FP64 A11 applies on a 40^3-element Q2 mesh (%d velocity dofs), %.1f s" % (n, P.nvel, el)}
 
 
-def solve_comparison(m, nlevels, rtol=1e-8, restart=30, max_it=600):
+def solve_comparison(m, nlevels, rtol=1e-8, restart=120, max_it=1000):
     """The viscous solve of the buoyancy problem (free-slip walls, 20 dense spheres with
     viscosity contrast 1e4) with GMG-preconditioned FGMRES, once with FP64 and once with FP32
     Chebyshev smoothers.  Wall time of each solve from x = 0 to rtol on the host clock
@@ -123,7 +123,7 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=30, max_it=600):
     return out
 
 
-def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=600, restart=30):
+def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1000, restart=120):
     """Independent buoyancy solves of uneven sizes (viscosity contrasts 1e2..1e6, seeded
     sphere placement, FP32 smoothing), run one after another and then concurrently (one
     stream and host thread per problem).  Host wall time of each variant."""

@@ -971,16 +971,23 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
     for (; j < restart && it < max_it; j++) {
       MGRC(ptq2_mg_vcycle(mg, mg->Vk[j], mg->Zk[j]));
       MGRC(ptq2_a11_apply(fc, mg->Zk[j], mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
-      // classical Gram-Schmidt with one re-orthogonalisation pass
+      // classical Gram-Schmidt; a second pass only when the first one cancelled more than
+      // half of w's norm (the DGKS criterion: ||w'|| < ||w|| / sqrt(2))
+      double wn;
+      MGRC(norm2(mg, mg->w, n, &wn));
       MGRC(mdot(mg, mg->Vptr, j + 1, mg->w, n, h.data()));
       MGCHK(hipMemcpyAsync(mg->hdev, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
       hipLaunchKernelGGL(k_maxpy, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Vptr, j + 1, (const double *)mg->hdev, mg->w);
-      MGRC(mdot(mg, mg->Vptr, j + 1, mg->w, n, h2.data()));
-      MGCHK(hipMemcpyAsync(mg->hdev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
-      hipLaunchKernelGGL(k_maxpy, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Vptr, j + 1, (const double *)mg->hdev, mg->w);
-      for (int i = 0; i <= j; i++) h[i] += h2[i];
       double hn;
       MGRC(norm2(mg, mg->w, n, &hn));
+      if (hn < 0.7071067811865476 * wn) {
+        MGRC(mdot(mg, mg->Vptr, j + 1, mg->w, n, h2.data()));
+        MGCHK(hipMemcpyAsync(mg->hdev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
+        hipLaunchKernelGGL(k_maxpy, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Vptr, j + 1, (const double *)mg->hdev,
+                           mg->w);
+        for (int i = 0; i <= j; i++) h[i] += h2[i];
+        MGRC(norm2(mg, mg->w, n, &hn));
+      }
       h[j + 1] = hn;
       if (hn > 0.0)
         hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->w, 1.0 / hn, mg->Vk[j + 1]);
@@ -1014,7 +1021,8 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
     }
     MGCHK(hipMemcpyAsync(mg->hdev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, mg->stream));
     hipLaunchKernelGGL(k_mcombine, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Zptr, j, (const double *)mg->hdev, x);
-    if (rn <= target) break;
+    // converged by the recurrence estimate: the next pass computes the true residual and
+    // stops there if it meets the target, else restarts from x
   }
   // true residual
   MGRC(ptq2_a11_apply(fc, x, mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
