@@ -242,7 +242,7 @@ def main():
         extra["batch"] = batch_comparison()
 
     traffic = None
-    prof = os.path.join(REPO, "profiles", "a11_fp64_128_traffic.json")
+    prof = os.path.join(REPO, "profiles", "a11_fp64g_128_traffic.json")
     if m == 128 and os.path.exists(prof):
         with open(prof) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
