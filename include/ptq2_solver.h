@@ -35,10 +35,13 @@ typedef struct {
   int esteig_its;       /* power iterations for the estimate of lambda_max(D^-1 A) */
   int use_graph;        /* 1: replay the V-cycle from a captured hipGraph */
   int coarse_max_dofs;  /* refuse a dense coarse level larger than this */
+  int galerkin_coarse;  /* 1: the coarsest operator is R A P of the level above it (assembled
+                           by probing), 0: rediscretised on the coarsest mesh */
 } ptq2_mg_opts;
 
 /* fill opts with the defaults: Chebyshev(4), FP32 smoothing, (0.2, 1.1) transform,
-   20 power iterations, graph replay, coarse dense limit 20000 dofs */
+   20 power iterations, graph replay, coarse dense limit 20000 dofs, Galerkin coarsest
+   operator */
 void ptq2_mg_default_opts(ptq2_mg_opts *opts);
 
 /* Build the hierarchy under a set-up context (mesh with BC mask + viscosity): coarse

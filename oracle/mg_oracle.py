@@ -191,14 +191,44 @@ def dense_inverse(L):
     return np.linalg.inv(A)
 
 
+def galerkin_inverse(F, C, P=None, R=None):
+    """Inverse of the Galerkin coarse operator Pi R A_F P Pi + (I - Pi) on level C (Pi zeroes
+    C's Dirichlet dofs; A_F with MatMult semantics), assembled column by column."""
+    P = P or prolong
+    R = R or restrict
+    n = C.n
+    A = np.zeros((n, n))
+    dmask = np.zeros(n, dtype=bool) if C.mask is None else C.mask > 0
+    for j in range(n):
+        if dmask[j]:
+            A[j, j] = 1.0
+            continue
+        e = np.zeros(n)
+        e[j] = 1.0
+        v = P(C.mx, C.my, C.mz, e)
+        if F.mask is not None:
+            v[F.mask > 0] = 0.0
+        col = R(C.mx, C.my, C.mz, F.matmult(v))
+        col[dmask] = 0.0
+        A[:, j] = col
+    A = 0.5 * (A + A.T)
+    return np.linalg.inv(A)
+
+
 class VCycle:
-    def __init__(self, levels, emin, emax, its, transfer="trilinear"):
+    """V-cycle of the GPU solver's hierarchy; galerkin=True makes the coarsest operator the
+    Galerkin product through the level above it (ptq2_mg_opts.galerkin_coarse)."""
+
+    def __init__(self, levels, emin, emax, its, transfer="trilinear", galerkin=False):
         self.lv = levels
         self.P = prolong_q2 if transfer == "q2" else prolong
         self.R = restrict_q2 if transfer == "q2" else restrict
         self.emin, self.emax, self.its = emin, emax, its
         self.dinv = [1.0 / L.diag() for L in levels[:-1]]
-        self.ainv = dense_inverse(levels[-1])
+        if galerkin and len(levels) >= 2:
+            self.ainv = galerkin_inverse(levels[-2], levels[-1], self.P, self.R)
+        else:
+            self.ainv = dense_inverse(levels[-1])
 
     def __call__(self, b, l=0):
         L = self.lv[l]

@@ -639,6 +639,7 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->esteig_its = 20;
   o->use_graph = 1;
   o->coarse_max_dofs = 20000;
+  o->galerkin_coarse = 1;
 }
 
 int ptq2_mg_destroy(ptq2_mg **pmg) {
@@ -831,9 +832,38 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     int rc = PTQ2_OK;
     hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, nc, np,
                        mg->ainv);
-    for (int j = 0; j < nc && rc == PTQ2_OK; j++) {
-      hipLaunchKernelGGL(k_unit, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, (int64_t)nc, (int64_t)j, e);
-      rc = ptq2_a11_apply(c, e, mg->ainv + (int64_t)j * np, PTQ2_DEVICE | PTQ2_MATMULT);
+    if (mg->opt.galerkin_coarse && mg->nl >= 2) {
+      // row j = Pi R A_f P Pi e_j (Pi zeroes coarse Dirichlet dofs, whose rows are e_j): the
+      // Galerkin product through the level above, probed one coarse dof at a time
+      const int lf = l - 1;
+      const int *mf = &mg->m[3 * lf], *mcc = &mg->m[3 * l];
+      const int NXf = 2 * mf[0] + 1, NYf = 2 * mf[1] + 1, NZf = 2 * mf[2] + 1;
+      const int NXc = 2 * mcc[0] + 1, NYc = 2 * mcc[1] + 1, NZc = 2 * mcc[2] + 1;
+      const int64_t nf = mg->n[lf];
+      std::vector<uint8_t> hm((size_t)nc, 0);
+      if (c->mask) MGCHK(hipMemcpy(hm.data(), c->mask, nc, hipMemcpyDeviceToHost));
+      double *vf = nullptr, *wf = nullptr;
+      MGRC(mg_alloc(mg, (void **)&vf, sizeof(double) * nf));
+      MGRC(mg_alloc(mg, (void **)&wf, sizeof(double) * nf));
+      const uint8_t *fmask = mg->ctx[lf]->mask;
+      for (int j = 0; j < nc && rc == PTQ2_OK; j++) {
+        double *row = mg->ainv + (int64_t)j * np;
+        hipLaunchKernelGGL(k_unit, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, (int64_t)nc, (int64_t)j,
+                           hm[j] ? row : e);
+        if (hm[j]) continue;
+        hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, nf, vf, 0.0);
+        hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, NXc, NYc, NXf, NYf, NZf,
+                           (const double *)e, fmask, vf);
+        rc = level_matmult<double>(mg, lf, vf, wf);
+        hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, NXf, NYf,
+                           NZf, (const double *)wf, (const double *)nullptr, (const uint8_t *)c->mask, row);
+      }
+      (void)hipFree(vf); (void)hipFree(wf);
+    } else {
+      for (int j = 0; j < nc && rc == PTQ2_OK; j++) {
+        hipLaunchKernelGGL(k_unit, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, (int64_t)nc, (int64_t)j, e);
+        rc = ptq2_a11_apply(c, e, mg->ainv + (int64_t)j * np, PTQ2_DEVICE | PTQ2_MATMULT);
+      }
     }
     (void)hipFree(e);
     MGRC(rc);

@@ -16,7 +16,7 @@ from ._lib import check
 class MGOpts(ctypes.Structure):
     _fields_ = [("nlevels", ctypes.c_int), ("smooth_its", ctypes.c_int), ("fp32_smoother", ctypes.c_int),
                 ("emin_frac", ctypes.c_double), ("emax_frac", ctypes.c_double), ("esteig_its", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int), ("galerkin_coarse", ctypes.c_int)]
 
 
 _SOLVER_SIGS = {
@@ -56,11 +56,13 @@ def default_opts():
 class ViscousSolver:
     """GMG V-cycle (hipGraph) + FGMRES for MatMult_MFStokes_A11 x = b on one GPU."""
 
-    def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000):
+    def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000,
+                 galerkin_coarse=True):
         L = _sl()
         o = default_opts()
         o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
         o.esteig_its, o.use_graph, o.coarse_max_dofs = esteig_its, 1 if use_graph else 0, coarse_max_dofs
+        o.galerkin_coarse = 1 if galerkin_coarse else 0
         self.ctx = ctx  # keep the fine context alive
         h = ctypes.c_void_p()
         check(L.ptq2_mg_create(ctypes.byref(h), ctx.handle, ctypes.byref(o)), "ptq2_mg_create", ctx.handle)

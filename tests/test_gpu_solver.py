@@ -26,15 +26,16 @@ def make(m, contrast, bc="freeslip", n_incl=3):
     return P, ctx
 
 
-def test_vcycle_fp64_matches_restatement():
+@pytest.mark.parametrize("galerkin", [False, True])
+def test_vcycle_fp64_matches_restatement(galerkin):
     m = 8
-    say("case: FP64 V-cycle 8^3 (3 levels) vs CPU restatement")
+    say("case: FP64 V-cycle 8^3 (3 levels, galerkin coarse %s) vs CPU restatement" % galerkin)
     P, ctx = make(m, 1e2)
-    S = ViscousSolver(ctx, nlevels=3, smooth_its=4, fp32=False, use_graph=True)
+    S = ViscousSolver(ctx, nlevels=3, smooth_its=4, fp32=False, use_graph=True, galerkin_coarse=galerkin)
     assert [tuple(r) for r in S.levels()] == [(8, 8, 8), (4, 4, 4), (2, 2, 2)]
     emin, emax = S.eigs()
     lv = G.hierarchy(m, m, m, P.coords, P.mask, P.eta, 3)
-    V = G.VCycle(lv, emin, emax, 4)
+    V = G.VCycle(lv, emin, emax, 4, galerkin=galerkin)
     r = M.uniform(3, P.nvel, -1, 1)
     r[P.mask > 0] = 0.0
     z = S.vcycle(torch.from_numpy(r).to(DEV)).cpu().numpy()
@@ -99,7 +100,7 @@ def test_vcycle_large_coarse_level():
     m = 12
     say("case: FP64 V-cycle 12^3 (2 levels, coarse 6591 dofs) vs CPU restatement")
     P, ctx = make(m, 1e3)
-    S = ViscousSolver(ctx, nlevels=2, smooth_its=2, fp32=False, use_graph=True)
+    S = ViscousSolver(ctx, nlevels=2, smooth_its=2, fp32=False, use_graph=True, galerkin_coarse=False)
     emin, emax = S.eigs()
     lv = G.hierarchy(m, m, m, P.coords, P.mask, P.eta, 2)
     V = G.VCycle(lv, emin, emax, 2)

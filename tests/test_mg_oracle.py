@@ -99,3 +99,28 @@ def test_vcycle_preconditions_gmres():
                         callback_type="pr_norm")
     assert info == 0 and len(hist) <= 25
     assert np.linalg.norm(b - lv[0].matmult(x)) <= 1e-8 * r0 * 1.01
+
+
+def test_galerkin_coarse_operator_is_rap():
+    """The Galerkin coarsest operator of the restatement equals P^T A_f P assembled from an
+    explicit trilinear prolongation matrix (restricted to interior coarse dofs)."""
+    m = 4
+    P = M.config_sinker(m, n_incl=3, contrast=1e2, bc="freeslip")
+    lv = G.hierarchy(m, m, m, P.coords, P.mask, P.eta, 2)
+    F, C = lv
+    nf, nc = F.n, C.n
+    Pm = np.zeros((nf, nc))
+    for j in range(nc):
+        e = np.zeros(nc)
+        e[j] = 1.0
+        Pm[:, j] = G.prolong(C.mx, C.my, C.mz, e)
+    Pm[F.mask > 0, :] = 0.0
+    Pm[:, C.mask > 0] = 0.0
+    Af = np.stack([F.matmult(np.eye(nf)[k]) for k in range(nf)], axis=1)
+    Ac = Pm.T @ Af @ Pm
+    Ac[C.mask > 0, C.mask > 0] = 1.0
+    inv = G.galerkin_inverse(F, C)
+    assert np.allclose(inv @ Ac, np.eye(nc), atol=1e-8)
+    # the restriction is the transpose of the prolongation
+    r = np.random.default_rng(0).standard_normal(nf)
+    assert np.allclose(G.restrict(C.mx, C.my, C.mz, r), (np.stack([G.prolong(C.mx, C.my, C.mz, np.eye(nc)[j]) for j in range(nc)], axis=1)).T @ r)
