@@ -230,9 +230,11 @@ __device__ __forceinline__ void load_tile2(const Box &b, const Tile &t, const Br
 // Write an LDS output tile back.  Bricks of one colour never share a node.  In
 // overwrite mode the first brick to touch a node (parity 0 along every axis on
 // which the node is shared, i.e. the lowest colour) stores, the later colours add.
-// The global values to add to are all loaded before the first store.
-template <typename T>
-__device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, const Tile &t, const T *__restrict__ s,
+// The global values to add to are all loaded before the first store.  The LDS tile may
+// be wider than the vector (an FP64 tile accumulating an FP32 operator): each node is
+// summed in the tile's type and rounded once.
+template <typename S, typename T>
+__device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, const Tile &t, const S *__restrict__ s,
                                            T *__restrict__ g, bool add_mode, int lane) {
   // opaque copy of the lane: keeps the compiler from sharing this tile's address
   // arithmetic with the loads at the top of the kernel (which would keep it live
@@ -257,21 +259,21 @@ __device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, cons
   for (int it = 0; it < TPAIRS; it++) {
     int y, z;
     tile_row(it, L, y, z);
-    if (L.col_ok && y < t.tny && z < t.tnz) g[tile_gidx(t, L, y, z)] = v[it] + s[L.lds_col + TRS * y + TPS * z];
+    if (L.col_ok && y < t.tny && z < t.tnz) g[tile_gidx(t, L, y, z)] = T(v[it] + s[L.lds_col + TRS * y + TPS * z]);
   }
 }
 
 // Uncoloured output: the whole LDS output tile of brick br to its slot of a tile
 // buffer, dense [c][z][y][x] with 9 nodes per edge (k_gather sums the slots).
 constexpr int TILEVALS = 3 * TN * TN * TN;
-template <typename T>
-__device__ __forceinline__ void store_tile_buf(const BrickId &br, const T *__restrict__ s, T *__restrict__ buf,
+template <typename S, typename T>
+__device__ __forceinline__ void store_tile_buf(const BrickId &br, const S *__restrict__ s, T *__restrict__ buf,
                                                int lane) {
   T *o = buf + (int64_t)br.lin * TILEVALS;
   for (int f = lane; f < TILEVALS; f += BEL) {
     const int c = f / (TN * TN * TN), r = f - c * (TN * TN * TN);
     const int z = r / (TN * TN), y = (r / TN) % TN, x = r % TN;
-    o[f] = s[c * TCS + z * TPS + y * TRS + tx(x)];
+    o[f] = T(s[c * TCS + z * TPS + y * TRS + tx(x)]);
   }
 }
 
@@ -486,6 +488,40 @@ __device__ __forceinline__ void acc_direct(const T dv[3][3], T RB[3][3][3], T RD
 // whole element body is one row-sized block of code (a fully unrolled element does
 // not fit the instruction cache).  Inside a row everything is unrolled.
 __device__ __forceinline__ void row_fence() { __builtin_amdgcn_sched_barrier(0); }
+
+// Asynchronous copy of NB contiguous bytes (a multiple of 16) from global memory into
+// LDS with global->LDS loads: lane l moves bytes [16l, 16l+16) of every 1 KiB chunk, and
+// no vector register is held while the copy is in flight.  Every lane of the wave must
+// take part.  Completion: s_waitcnt vmcnt(0) (WAIT_VM0).
+template <typename T>
+__device__ __forceinline__ const T *uniform_ptr(const T *p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (const T *)(((uint64_t)hi << 32) | lo);
+}
+template <int NB>
+__device__ __forceinline__ void lds_copy_async(const void *src, void *lds, int lane) {
+  static_assert(NB % 16 == 0, "16-byte chunks");
+  constexpr int CH = NB / 16, FULL = CH / 64, REM = CH % 64;
+  const char *s0 = uniform_ptr((const char *)src);  // scalar base, per-lane byte offset
+  char *d0 = (char *)lds;
+  int off = 16 * lane;
+  asm volatile("" : "+v"(off));  // one fresh vector register per copy site
+#pragma unroll
+  for (int i = 0; i < FULL; i++)
+    __builtin_amdgcn_global_load_lds((const void *)(s0 + i * 1024 + off),
+                                     (__attribute__((address_space(3))) void *)(d0 + i * 1024), 16, 0, 0);
+  if (REM) {
+    if (lane < REM)
+      __builtin_amdgcn_global_load_lds((const void *)(s0 + FULL * 1024 + off),
+                                       (__attribute__((address_space(3))) void *)(d0 + FULL * 1024), 16, 0, 0);
+  }
+}
+// s_waitcnt immediates (gfx9 encoding): all vector-memory operations done (the
+// global->LDS copies included) / all LDS operations done
+constexpr int WAIT_VM0 = 0x0F70;
+constexpr int WAIT_LGKM0 = 0xC07F;
 
 struct Coef1 {
   double b[3], d[3];

@@ -53,9 +53,9 @@ __device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, in
   row_fence();
 }
 
-// add the element's 81 outputs into the LDS output tile
-template <typename T>
-__device__ __forceinline__ void emit_element(T *Ys, int base, const T Y[3][3][3][3]) {
+// add the element's 81 outputs into the LDS output tile (of the element's type or wider)
+template <typename S, typename T>
+__device__ __forceinline__ void emit_element(S *Ys, int base, const T Y[3][3][3][3]) {
 #pragma unroll
   for (int c = 0; c < 3; c++)
 #pragma unroll
@@ -63,7 +63,7 @@ __device__ __forceinline__ void emit_element(T *Ys, int base, const T Y[3][3][3]
 #pragma unroll
       for (int j = 0; j < 3; j++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) lds_add(&Ys[base + toff(c, i, j, k)], Y[c][k][j][i]);
+        for (int i = 0; i < 3; i++) lds_add(&Ys[base + toff(c, i, j, k)], S(Y[c][k][j][i]));
 }
 
 __device__ __forceinline__ void load9(const double *__restrict__ ep, int qc, double *e9) {
@@ -182,11 +182,19 @@ __device__ __forceinline__ void slab_z32(const float *Us, int base, const Coef1 
 __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
                                                   const uint8_t *__restrict__ mask, const float *__restrict__ fac,
                                                   float *__restrict__ y, int add_mode, float *__restrict__ tilebuf) {
-  __shared__ float Us[3 * TCS];
+  // LDS: the FP32 velocity tile and one quadrature row of factors ([qa*9+m][lane]) during
+  // the rows, then the output tile in FP64 (LDS float atomics are many times slower than
+  // ds_add_f64 on gfx950, and the wider sum costs nothing in the result)
+  __shared__ double Ts[3 * TCS];
+  float *Us = reinterpret_cast<float *>(Ts);
+  constexpr int FROW = 27 * BEL;
+  float *Fs = Us + 3 * TCS;
+  static_assert(sizeof(float) * (3 * TCS + FROW) <= sizeof(Ts), "velocity tile + factor row fit the output tile");
   const int lane = threadIdx.x;
   const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_lin(b, blockIdx.x);
   const Tile t = tile_of(b, br);
-  const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
+  const float *fsrc = fac + (int64_t)br.lin * 243 * BEL;
+  lds_copy_async<FROW * 4>(fsrc, Fs, lane);
   load_tile(b, t, br, u, mask, Us, lane);
   __syncthreads();
   const int base = lane_base(lane);
@@ -200,20 +208,25 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
       for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
-  if (active) {
-    // the first point's factors of row r+1 are loaded while row r's outputs are
-    // accumulated; the other two points' factors at the start of their row
-    float F0[9];
-#pragma unroll
-    for (int n = 0; n < 9; n++) F0[n] = fbase[(int64_t)n * BEL];
+  // Row r+1's factors travel HBM -> LDS while row r computes.  Every lane runs the rows
+  // (the copies need the whole wave); the inactive lanes of a partial brick compute on
+  // tile and factor slots that no element owns and their outputs are discarded.
+  {
     float ZB[3][3][3], ZD[3][3][3];
 #pragma nounroll
     for (int row = 0; row < 9; row++) {
       const int qb = row % 3, qc = row / 3;
       const Coef1 cb = coef1(qb), cc = coef1(qc);
-      float F12[18];
+      __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+      float F0[9], F12[18];
 #pragma unroll
-      for (int n = 0; n < 18; n++) F12[n] = fbase[(int64_t)(27 * row + 9 + n) * BEL];
+      for (int n = 0; n < 9; n++) F0[n] = Fs[n * BEL + lane];
+#pragma unroll
+      for (int n = 0; n < 18; n++) F12[n] = Fs[(9 + n) * BEL + lane];
+      if (row < 8) {
+        __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+        lds_copy_async<FROW * 4>(fsrc + (row + 1) * FROW, Fs, lane);
+      }
       if (qb == 0) slab_z32(Us, base, cc, ZB, ZD);
       float ur0[3][3], ur1[3][3], ur2[3][3];
 #pragma unroll
@@ -230,23 +243,19 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
       a11_qp32<1>(ur0, ur1, ur2, F12, P0, P1, P2);
       a11_qp32<2>(ur0, ur1, ur2, F12 + 9, P0, P1, P2);
       row_fence();
-      if (row < 8) {
-#pragma unroll
-        for (int n = 0; n < 9; n++) F0[n] = fbase[(int64_t)(27 * (row + 1) + n) * BEL];
-      }
       acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
       row_fence();
     }
   }
   __syncthreads();
-  zero_tile(Us, lane);
+  zero_tile(Ts, lane);
   __syncthreads();
-  if (active) emit_element(Us, base, Y);
+  if (active) emit_element(Ts, base, Y);
   __syncthreads();
   if (color >= 0)
-    store_tile(b, br, t, Us, y, add_mode != 0, lane);
+    store_tile(b, br, t, Ts, y, add_mode != 0, lane);
   else
-    store_tile_buf(br, Us, tilebuf, lane);
+    store_tile_buf(br, Ts, tilebuf, lane);
 }
 
 // ------------------------------------------------------ A11 FP64, stored geometry
@@ -422,7 +431,8 @@ __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const flo
                                                      const float *__restrict__ p, const uint8_t *__restrict__ mask,
                                                      const float *__restrict__ fac, const float *__restrict__ facb,
                                                      float *__restrict__ yu, float *__restrict__ yp, int add_mode) {
-  __shared__ float Us[3 * TCS];
+  __shared__ double Ts[3 * TCS];  // FP32 velocity tile during the rows, FP64 output tile after
+  float *Us = reinterpret_cast<float *>(Ts);
   const int lane = threadIdx.x;
   const BrickId br = brick_of(b, color, blockIdx.x);
   const Tile t = tile_of(b, br);
@@ -479,11 +489,11 @@ __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const flo
     for (int i = 0; i < 4; i++) o[i] = add_mode ? o[i] + Yp[i] : Yp[i];
   }
   __syncthreads();
-  zero_tile(Us, lane);
+  zero_tile(Ts, lane);
   __syncthreads();
-  if (active) emit_element(Us, base, Y);
+  if (active) emit_element(Ts, base, Y);
   __syncthreads();
-  store_tile(b, br, t, Us, yu, add_mode != 0, lane);
+  store_tile(b, br, t, Ts, yu, add_mode != 0, lane);
 }
 
 // ------------------------------------------------------- pressure basis helper
