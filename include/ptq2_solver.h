@@ -37,11 +37,16 @@ typedef struct {
   int coarse_max_dofs;  /* refuse a dense coarse level larger than this */
   int galerkin_coarse;  /* 1: the coarsest operator is R A P of the level above it (assembled
                            by probing), 0: rediscretised on the coarsest mesh */
+  int coarse_eta_mean;  /* PTQ2_ETA_MEAN_*: how a coarse quadrature point combines the fine
+                           quadrature viscosities of the child elements it touches (mean or
+                           maximum) */
 } ptq2_mg_opts;
+
+enum { PTQ2_ETA_MEAN_ARITHMETIC = 0, PTQ2_ETA_MEAN_GEOMETRIC = 1, PTQ2_ETA_MEAN_HARMONIC = 2, PTQ2_ETA_MEAN_MAX = 3 };
 
 /* fill opts with the defaults: Chebyshev(4), FP32 smoothing, (0.2, 1.1) transform,
    20 power iterations, graph replay, coarse dense limit 20000 dofs, Galerkin coarsest
-   operator */
+   operator, arithmetic coarse viscosity */
 void ptq2_mg_default_opts(ptq2_mg_opts *opts);
 
 /* Build the hierarchy under a set-up context (mesh with BC mask + viscosity): coarse

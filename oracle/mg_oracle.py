@@ -7,8 +7,8 @@ TEST INFRASTRUCTURE ONLY (the checker of tests/test_gpu_solver.py).  Follows:
   - trilinear interpolation of the Q2 node lattice (PETSc DMDA Q1 interpolation,
     src/ptatin_driver_linear_ts.c:996-1006) and restriction = its transpose, with the
     corrections and restricted residuals zeroed on Dirichlet dofs;
-  - coarse quadrature-point viscosity = mean of the fine quadrature values of the child
-    elements a coarse point touches;
+  - coarse quadrature-point viscosity = mean (arithmetic, geometric or harmonic) or maximum of
+    the fine quadrature values of the child elements a coarse point touches;
   - an exact (dense) solve on the coarsest level.
 The element operators are the C oracle's (oracle/ptq2_oracle.c).
 """
@@ -18,9 +18,14 @@ from . import oracle as O
 
 
 def eta_coarsen(mx, my, mz, eta, mode="arithmetic"):
-    """coarse (mx/2,...) quadrature values from the fine ones (arithmetic or geometric mean)"""
+    """coarse (mx/2,...) quadrature values from the fine ones (arithmetic, geometric or
+    harmonic mean)"""
     if mode == "geometric":
         return np.exp(eta_coarsen(mx, my, mz, np.log(eta), "arithmetic"))
+    if mode == "harmonic":
+        return 1.0 / eta_coarsen(mx, my, mz, 1.0 / np.asarray(eta), "arithmetic")
+    if mode == "max":
+        return _eta_coarsen_max(mx, my, mz, eta)
     e = np.asarray(eta).reshape(mz, my, mx, 3, 3, 3)  # (ez,ey,ex, qc,qb,qa)
     mxc, myc, mzc = mx // 2, my // 2, mz // 2
     out = np.zeros((mzc, myc, mxc, 3, 3, 3))
@@ -37,6 +42,22 @@ def eta_coarsen(mx, my, mz, eta, mode="arithmetic"):
                         for a in as_:
                             acc = acc + child_mean[:, c, :, b, :, a]
                 out[:, :, :, qc, qb, qa] = acc / (len(cs) * len(bs) * len(as_))
+    return out.reshape(-1)
+
+
+def _eta_coarsen_max(mx, my, mz, eta):
+    e = np.asarray(eta).reshape(mz, my, mx, 3, 3, 3)
+    mxc, myc, mzc = mx // 2, my // 2, mz // 2
+    child_max = e.reshape(mzc, 2, myc, 2, mxc, 2, 27).max(axis=6)
+    out = np.zeros((mzc, myc, mxc, 3, 3, 3))
+    for qc in range(3):
+        cs = [0] if qc == 0 else ([1] if qc == 2 else [0, 1])
+        for qb in range(3):
+            bs = [0] if qb == 0 else ([1] if qb == 2 else [0, 1])
+            for qa in range(3):
+                as_ = [0] if qa == 0 else ([1] if qa == 2 else [0, 1])
+                out[:, :, :, qc, qb, qa] = np.max([child_max[:, c, :, b, :, a] for c in cs for b in bs for a in as_],
+                                                  axis=0)
     return out.reshape(-1)
 
 

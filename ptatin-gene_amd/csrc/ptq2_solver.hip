@@ -27,7 +27,7 @@ namespace {
 
 constexpr int TPB = 256;
 // largest FGMRES restart length (Krylov pointer tables and reduction buffers are sized
-// for it; slot MAXRESTART of the Z table is a scratch pointer for norms)
+// for it)
 constexpr int MAXRESTART = 256;
 
 inline unsigned nblk(int64_t n, int64_t cap = 16384) {
@@ -153,10 +153,11 @@ __global__ void k_dense_gemv(int n, int ld, const double *__restrict__ Ainv, con
   if (lane == 0) x[row] = static_cast<T>(s);
 }
 
-// coarse quadrature-point viscosity: the mean of the fine quadrature values of the
-// children that each coarse quadrature point touches (1, 2, 4 or 8 children).
+// coarse quadrature-point viscosity: the mean (arithmetic, geometric or harmonic) or the
+// maximum of the fine quadrature values of the children that each coarse quadrature point touches (1, 2,
+// 4 or 8 children).
 __global__ void k_eta_coarsen(Box f, int mxc, int myc, int mzc, const double *__restrict__ eta_bm_f,
-                              double *__restrict__ eta_c) {
+                              int mean, double *__restrict__ eta_c) {
   int64_t nq = 27 * (int64_t)mxc * myc * mzc;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += (int64_t)gridDim.x * blockDim.x) {
     int q = (int)(t % 27);
@@ -175,55 +176,224 @@ __global__ void k_eta_coarsen(Box f, int mxc, int myc, int mzc, const double *__
           int64_t lin = (ex >> 2) + (int64_t)f.nbx * ((ey >> 2) + (int64_t)f.nby * (ez >> 2));
           int lane = ptq2::elem_lane(ex & 3, ey & 3, ez & 3);
           const double *src = eta_bm_f + lin * 27 * ptq2::BEL + lane;
-          for (int qq = 0; qq < 27; qq++) s += src[qq * ptq2::BEL];
+          for (int qq = 0; qq < 27; qq++) {
+            const double v = src[qq * ptq2::BEL];
+            if (mean == PTQ2_ETA_MEAN_MAX)
+              s = fmax(s, v);
+            else
+              s += mean == PTQ2_ETA_MEAN_GEOMETRIC ? log(v) : (mean == PTQ2_ETA_MEAN_HARMONIC ? 1.0 / v : v);
+          }
           cnt += 27;
         }
       }
     }
-    eta_c[t] = s / cnt;
+    const double a = mean == PTQ2_ETA_MEAN_MAX ? s : s / cnt;
+    eta_c[t] = mean == PTQ2_ETA_MEAN_GEOMETRIC ? exp(a) : (mean == PTQ2_ETA_MEAN_HARMONIC ? 1.0 / a : a);
   }
 }
 
-// partial dot products of w with nv vectors: part[v * nb + block].  The vectors are taken
-// MD at a time, so w is read once per group instead of once per vector.
-constexpr int MD = 8;
-__global__ void k_mdot(int64_t n, const double *const *__restrict__ V, int nv, const double *__restrict__ w,
-                       double *__restrict__ part) {
-  __shared__ double red[MD][TPB / 64];
-  for (int v0 = 0; v0 < nv; v0 += MD) {
-    const int nm = min(MD, nv - v0);
-    double s[MD];
+// Dot products and updates of the Gram-Schmidt passes.  The reductions run on a fixed
+// grid of MDB blocks: every block writes one partial per product and k_sum_parts adds a
+// product's partials in block order, so every sum is reproducible run to run.
+constexpr int MD = 8;      // vectors per pass over w
+constexpr int MDB = 2048;  // blocks of the reduction kernels (partials per product)
+constexpr int MDU = 4;     // grid strides per loop iteration (independent loads in flight)
+
+template <int NR>
+__device__ __forceinline__ void block_partials(double (&s)[NR], double (*red)[TPB / 64]) {
 #pragma unroll
-    for (int m = 0; m < MD; m++) s[m] = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int m = 0; m < NR; m++) {
+    for (int off = 32; off > 0; off >>= 1) s[m] += __shfl_down(s[m], off, 64);
+    if ((threadIdx.x & 63) == 0) red[m][threadIdx.x / 64] = s[m];
+  }
+}
+
+// part[v * MDB + block]: partial of V[v].w for v < nv; with self, part[nv * MDB + block]:
+// partial of w.w (taken in the first pass over w)
+__global__ __launch_bounds__(TPB) void k_mdot(int64_t n, const double *const *__restrict__ V, int nv,
+                                              const double *__restrict__ w, int self, double *__restrict__ part) {
+  __shared__ double red[MD + 1][TPB / 64];
+  const int64_t stride = (int64_t)MDB * TPB;
+  const int64_t i0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int ngroups = nv > 0 ? (nv + MD - 1) / MD : 1;
+  for (int gi = 0; gi < ngroups; gi++) {
+    const int v0 = gi * MD;
+    const int nm = max(0, min(MD, nv - v0));
+    const bool ws = self && gi == 0;
+    double s[MD + 1];
+    const double *vp[MD];
+#pragma unroll
+    for (int m = 0; m <= MD; m++) s[m] = 0.0;
+#pragma unroll
+    for (int m = 0; m < MD; m++) vp[m] = m < nm ? V[v0 + m] : w;
+    int64_t i = i0;
+    for (; i + (MDU - 1) * stride < n; i += MDU * stride) {
+      double wv[MDU];
+#pragma unroll
+      for (int u = 0; u < MDU; u++) wv[u] = w[i + u * stride];
+#pragma unroll
+      for (int m = 0; m < MD; m++)
+        if (m < nm) {
+          double x[MDU];
+#pragma unroll
+          for (int u = 0; u < MDU; u++) x[u] = vp[m][i + u * stride];
+#pragma unroll
+          for (int u = 0; u < MDU; u++) s[m] = fma(x[u], wv[u], s[m]);
+        }
+      if (ws)
+#pragma unroll
+        for (int u = 0; u < MDU; u++) s[MD] = fma(wv[u], wv[u], s[MD]);
+    }
+    for (; i < n; i += stride) {
       const double wi = w[i];
 #pragma unroll
       for (int m = 0; m < MD; m++)
-        if (m < nm) s[m] += V[v0 + m][i] * wi;
+        if (m < nm) s[m] = fma(vp[m][i], wi, s[m]);
+      if (ws) s[MD] = fma(wi, wi, s[MD]);
     }
-#pragma unroll
-    for (int m = 0; m < MD; m++) {
-      for (int off = 32; off > 0; off >>= 1) s[m] += __shfl_down(s[m], off, 64);
-      if ((threadIdx.x & 63) == 0) red[m][threadIdx.x / 64] = s[m];
-    }
+    block_partials<MD + 1>(s, red);
     __syncthreads();
-    if (threadIdx.x < nm) {
-      double t = 0.0;
-      for (int k = 0; k < TPB / 64; k++) t += red[threadIdx.x][k];
-      part[(int64_t)(v0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    const int t = threadIdx.x;
+    if (t < nm || (ws && t == MD)) {
+      const double r = red[t][0] + red[t][1] + red[t][2] + red[t][3];
+      part[(int64_t)(t < nm ? v0 + t : nv) * MDB + blockIdx.x] = r;
     }
     __syncthreads();
   }
 }
 
-// w -= sum_v h[v] V[v]
-__global__ void k_maxpy(int64_t n, const double *const *__restrict__ V, int nv, const double *__restrict__ h,
-                        double *__restrict__ w) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+// out[p] = sum of the MDB partials of product p, in block order (one block per product)
+__global__ __launch_bounds__(TPB) void k_sum_parts(const double *__restrict__ part, double *__restrict__ out) {
+  __shared__ double red[1][TPB / 64];
+  const double *p = part + (int64_t)blockIdx.x * MDB;
+  double s[1] = {0.0};
+  for (int k = threadIdx.x; k < MDB; k += TPB) s[0] += p[k];
+  block_partials<1>(s, red);
+  __syncthreads();
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+}
+
+// w -= sum_v h[v] V[v], and part[block]: partial of the updated w.w
+__global__ __launch_bounds__(TPB) void k_maxpy_norm(int64_t n, const double *const *__restrict__ V, int nv,
+                                                    const double *__restrict__ h, double *__restrict__ w,
+                                                    double *__restrict__ part) {
+  __shared__ double red[1][TPB / 64];
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)MDB * TPB) {
     double s = w[i];
     for (int v = 0; v < nv; v++) s -= h[v] * V[v][i];
     w[i] = s;
+    acc[0] = fma(s, s, acc[0]);
   }
+  block_partials<1>(acc, red);
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+}
+
+// Gram-Schmidt with delayed reorthogonalisation (CGS2 in two passes per Arnoldi step): the
+// pending candidate u (stored unnormalised, u = su * slot) is reorthogonalised against the
+// finished basis V in the same pass that projects the new vector w.
+// Pass 1, part rows: 2v, 2v+1 -> V[v].u', V[v].w (v < nv); 2nv, 2nv+1, 2nv+2 -> u'.w, u'.u',
+// w.w, with u' the stored (unscaled) candidate.  MD2 basis vectors per pass over u' and w.
+constexpr int MD2 = 16;
+__global__ __launch_bounds__(TPB) void k_mdot2(int64_t n, const double *const *__restrict__ V, int nv,
+                                               const double *__restrict__ u, const double *__restrict__ w,
+                                               double *__restrict__ part) {
+  constexpr int NR = 2 * MD2 + 3;
+  __shared__ double red[NR][TPB / 64];
+  const int64_t stride = (int64_t)MDB * TPB;
+  const int64_t i0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const int ngroups = nv > 0 ? (nv + MD2 - 1) / MD2 : 1;
+  for (int gi = 0; gi < ngroups; gi++) {
+    const int v0 = gi * MD2;
+    const int nm = max(0, min(MD2, nv - v0));
+    const bool first = gi == 0;
+    double s[NR];
+    const double *vp[MD2];
+#pragma unroll
+    for (int m = 0; m < NR; m++) s[m] = 0.0;
+#pragma unroll
+    for (int m = 0; m < MD2; m++) vp[m] = m < nm ? V[v0 + m] : u;
+    int64_t i = i0;
+    for (; i + (MDU - 1) * stride < n; i += MDU * stride) {
+      double uv[MDU], wv[MDU];
+#pragma unroll
+      for (int q = 0; q < MDU; q++) {
+        uv[q] = u[i + q * stride];
+        wv[q] = w[i + q * stride];
+      }
+#pragma unroll
+      for (int m = 0; m < MD2; m++)
+        if (m < nm) {
+          double x[MDU];
+#pragma unroll
+          for (int q = 0; q < MDU; q++) x[q] = vp[m][i + q * stride];
+#pragma unroll
+          for (int q = 0; q < MDU; q++) {
+            s[2 * m] = fma(x[q], uv[q], s[2 * m]);
+            s[2 * m + 1] = fma(x[q], wv[q], s[2 * m + 1]);
+          }
+        }
+      if (first)
+#pragma unroll
+        for (int q = 0; q < MDU; q++) {
+          s[2 * MD2] = fma(uv[q], wv[q], s[2 * MD2]);
+          s[2 * MD2 + 1] = fma(uv[q], uv[q], s[2 * MD2 + 1]);
+          s[2 * MD2 + 2] = fma(wv[q], wv[q], s[2 * MD2 + 2]);
+        }
+    }
+    for (; i < n; i += stride) {
+      const double ui = u[i], wi = w[i];
+#pragma unroll
+      for (int m = 0; m < MD2; m++)
+        if (m < nm) {
+          const double x = vp[m][i];
+          s[2 * m] = fma(x, ui, s[2 * m]);
+          s[2 * m + 1] = fma(x, wi, s[2 * m + 1]);
+        }
+      if (first) {
+        s[2 * MD2] = fma(ui, wi, s[2 * MD2]);
+        s[2 * MD2 + 1] = fma(ui, ui, s[2 * MD2 + 1]);
+        s[2 * MD2 + 2] = fma(wi, wi, s[2 * MD2 + 2]);
+      }
+    }
+    block_partials<NR>(s, red);
+    __syncthreads();
+    const int t = threadIdx.x;
+    int row = -1;
+    if (t < 2 * nm) row = 2 * v0 + t;
+    else if (first && t >= 2 * MD2 && t < NR) row = 2 * nv + (t - 2 * MD2);
+    if (row >= 0) part[(int64_t)row * MDB + blockIdx.x] = red[t][0] + red[t][1] + red[t][2] + red[t][3];
+    __syncthreads();
+  }
+}
+
+// Pass 2: with uu = su * u' and the coefficient table ab = [a (nv), b (nv)]:
+//   u  <- (uu - sum_v a[v] V[v]) * ia      (the finished basis vector, written over u')
+//   wo <- w - sum_v b[v] V[v] - gam * uu   (the next candidate, unnormalised)
+// and part[block]: partial of wo.wo.
+__global__ __launch_bounds__(TPB) void k_dcgs_update(int64_t n, const double *const *__restrict__ V, int nv,
+                                                     const double *__restrict__ ab, double su, double ia, double gam,
+                                                     double *__restrict__ u, const double *__restrict__ w,
+                                                     double *__restrict__ wo, double *__restrict__ part) {
+  __shared__ double red[1][TPB / 64];
+  double acc[1] = {0.0};
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)MDB * TPB) {
+    double sa = 0.0, sb = 0.0;
+    for (int v = 0; v < nv; v++) {
+      const double x = V[v][i];
+      sa = fma(ab[v], x, sa);
+      sb = fma(ab[nv + v], x, sb);
+    }
+    const double uu = su * u[i];
+    u[i] = (uu - sa) * ia;
+    const double r = w[i] - sb - gam * uu;
+    wo[i] = r;
+    acc[0] = fma(r, r, acc[0]);
+  }
+  block_partials<1>(acc, red);
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
 }
 
 // x += sum_v h[v] V[v]
@@ -312,7 +482,7 @@ struct ptq2_mg {
   hipGraphExec_t exec = nullptr;
   // Krylov workspace
   std::vector<double *> Vk, Zk;
-  double *w = nullptr, *part = nullptr, *hdev = nullptr;
+  double *w = nullptr, *part = nullptr, *psum = nullptr, *hdev = nullptr;
   const double **Vptr = nullptr;  // device array of pointers (Vk)
   const double **Zptr = nullptr;
   std::vector<double> hist;
@@ -452,28 +622,58 @@ int vcycle_body(ptq2_mg *mg) {
   return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle");
 }
 
-// deterministic dot products: per-block partials summed on the host in a fixed order
-int mdot(ptq2_mg *mg, const double **Vdev, int nv, const double *w, int64_t n, double *out) {
-  const unsigned nb = 1024;
-  hipLaunchKernelGGL(k_mdot, dim3(nb), dim3(TPB), 0, mg->stream, n, Vdev, nv, w, mg->part);
-  std::vector<double> host((size_t)nv * nb);
-  MGCHK(hipMemcpyAsync(host.data(), mg->part, sizeof(double) * nv * nb, hipMemcpyDeviceToHost, mg->stream));
+// Reproducible dot products out[v] = V[v].w (v < nv) and, with self, out[nv] = w.w, in one
+// pass over w per MD vectors; the partials are summed on the device in a fixed order.
+int mdot(ptq2_mg *mg, const double *const *Vdev, int nv, const double *w, int64_t n, bool self, double *out) {
+  const int np = nv + (self ? 1 : 0);
+  if (np == 0) return PTQ2_OK;
+  hipLaunchKernelGGL(k_mdot, dim3(MDB), dim3(TPB), 0, mg->stream, n, Vdev, nv, w, self ? 1 : 0, mg->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(np), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
+  MGCHK(hipMemcpyAsync(out, mg->psum, sizeof(double) * np, hipMemcpyDeviceToHost, mg->stream));
   MGCHK(hipStreamSynchronize(mg->stream));
-  for (int v = 0; v < nv; v++) {
-    double s = 0.0;
-    for (unsigned b = 0; b < nb; b++) s += host[(size_t)v * nb + b];
-    out[v] = s;
-  }
+  return PTQ2_OK;
+}
+
+// pass 1 of the delayed-reorthogonalisation step: out[2v], out[2v+1] = V[v].u', V[v].w (v < nv),
+// out[2nv..2nv+2] = u'.w, u'.u', w.w
+int mdot2(ptq2_mg *mg, int nv, const double *u, const double *w, int64_t n, double *out) {
+  const int np = 2 * nv + 3;
+  hipLaunchKernelGGL(k_mdot2, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv, u, w, mg->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(np), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
+  MGCHK(hipMemcpyAsync(out, mg->psum, sizeof(double) * np, hipMemcpyDeviceToHost, mg->stream));
+  MGCHK(hipStreamSynchronize(mg->stream));
+  return PTQ2_OK;
+}
+
+// pass 2 (coefficients [a, b] already in hdev); returns ||wo||
+int dcgs_update(ptq2_mg *mg, int nv, double su, double ia, double gam, double *u, double *wo, int64_t n,
+                double *nrm) {
+  hipLaunchKernelGGL(k_dcgs_update, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv, (const double *)mg->hdev,
+                     su, ia, gam, u, (const double *)mg->w, wo, mg->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
+  double d;
+  MGCHK(hipMemcpyAsync(&d, mg->psum, sizeof(double), hipMemcpyDeviceToHost, mg->stream));
+  MGCHK(hipStreamSynchronize(mg->stream));
+  *nrm = std::sqrt(std::max(d, 0.0));
   return PTQ2_OK;
 }
 
 int norm2(ptq2_mg *mg, const double *x, int64_t n, double *out) {
-  // a pointer table entry pointing at x itself, stored after the Krylov tables
-  const double **slot = mg->Zptr + MAXRESTART;
-  MGCHK(hipMemcpyAsync((void *)slot, &x, sizeof(double *), hipMemcpyHostToDevice, mg->stream));
   double d;
-  MGRC(mdot(mg, slot, 1, x, n, &d));
+  MGRC(mdot(mg, nullptr, 0, x, n, true, &d));
   *out = std::sqrt(std::max(d, 0.0));
+  return PTQ2_OK;
+}
+
+// t -= sum_{v<nv} hdev[v] V[v] and the 2-norm of the result, in one pass
+int maxpy_norm(ptq2_mg *mg, int nv, double *t, int64_t n, double *nrm) {
+  hipLaunchKernelGGL(k_maxpy_norm, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv, (const double *)mg->hdev,
+                     t, mg->part);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
+  double d;
+  MGCHK(hipMemcpyAsync(&d, mg->psum, sizeof(double), hipMemcpyDeviceToHost, mg->stream));
+  MGCHK(hipStreamSynchronize(mg->stream));
+  *nrm = std::sqrt(std::max(d, 0.0));
   return PTQ2_OK;
 }
 
@@ -625,6 +825,19 @@ __global__ void k_negate(int64_t n, double *__restrict__ x) {
 }
 
 
+// z = M^-1 (scale r): one V-cycle on the scaled input
+int vcycle_scaled(ptq2_mg *mg, const double *r, double scale, double *z) {
+  const int64_t n = mg->n[0];
+  if (scale == 1.0)
+    MGCHK(hipMemcpyAsync(mg->r_in, r, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  else
+    hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, r, scale, mg->r_in);
+  if (mg->exec) MGCHK(hipGraphLaunch(mg->exec, mg->stream));
+  else MGRC(vcycle_body(mg));
+  MGCHK(hipMemcpyAsync(z, mg->z_out, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+  return PTQ2_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -640,6 +853,7 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->use_graph = 1;
   o->coarse_max_dofs = 20000;
   o->galerkin_coarse = 1;
+  o->coarse_eta_mean = PTQ2_ETA_MEAN_ARITHMETIC;
 }
 
 int ptq2_mg_destroy(ptq2_mg **pmg) {
@@ -657,7 +871,7 @@ int ptq2_mg_destroy(ptq2_mg **pmg) {
   if (!mg->ctx.empty() && mg->ctx[0]) ptq2_set_stream(mg->ctx[0], mg->fine_stream0);
   for (double *p : mg->Vk) (void)hipFree(p);
   for (double *p : mg->Zk) (void)hipFree(p);
-  for (void *p : {(void *)mg->ainv, (void *)mg->r_in, (void *)mg->z_out, (void *)mg->w, (void *)mg->part,
+  for (void *p : {(void *)mg->ainv, (void *)mg->r_in, (void *)mg->z_out, (void *)mg->w, (void *)mg->part, (void *)mg->psum,
                   (void *)mg->hdev, (void *)mg->Vptr, (void *)mg->Zptr})
     if (p) (void)hipFree(p);
   delete mg;
@@ -675,7 +889,8 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   ptq2_mg_opts o;
   if (opts) o = *opts;
   else ptq2_mg_default_opts(&o);
-  if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0))
+  if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0) ||
+      o.coarse_eta_mean < PTQ2_ETA_MEAN_ARITHMETIC || o.coarse_eta_mean > PTQ2_ETA_MEAN_MAX)
     return PTQ2_ERR_INVALID_ARG;
   for (int l = 1; l < o.nlevels; l++) {
     int f = 1 << l;
@@ -731,7 +946,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     MGRC(mg_alloc(mg, (void **)&etac, sizeof(double) * 27 * (size_t)c->nel));
     ptq2_ctx *p = mg->ctx[l - 1];
     hipLaunchKernelGGL(k_eta_coarsen, dim3(nblk(27 * c->nel)), dim3(TPB), 0, mg->stream, p->box, mx, my, mz,
-                       (const double *)p->eta_bm, etac);
+                       (const double *)p->eta_bm, o.coarse_eta_mean, etac);
     int rc = ptq2_set_viscosity(c, etac, 0, PTQ2_DEVICE);
     (void)hipFree(etac);
     MGRC(rc);
@@ -743,8 +958,9 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   if (mg->n.back() > o.coarse_max_dofs) return mgfail(mg, PTQ2_ERR_SIZE, "coarse level too large for the dense solve");
 
   // ---- small reduction buffers (also used by the solve)
-  MGRC(mg_alloc(mg, (void **)&mg->part, sizeof(double) * (MAXRESTART + 1) * 1024));
-  MGRC(mg_alloc(mg, (void **)&mg->hdev, sizeof(double) * (MAXRESTART + 1)));
+  MGRC(mg_alloc(mg, (void **)&mg->part, sizeof(double) * (2 * MAXRESTART + 4) * MDB));
+  MGRC(mg_alloc(mg, (void **)&mg->psum, sizeof(double) * (2 * MAXRESTART + 4)));
+  MGRC(mg_alloc(mg, (void **)&mg->hdev, sizeof(double) * (2 * MAXRESTART + 2)));
   MGRC(mg_alloc(mg, (void **)&mg->Vptr, sizeof(double *) * (MAXRESTART + 1)));
   MGRC(mg_alloc(mg, (void **)&mg->Zptr, sizeof(double *) * (MAXRESTART + 1)));
 
@@ -936,12 +1152,7 @@ int ptq2_mg_set_stream(ptq2_mg *mg, void *s) {
 
 int ptq2_mg_vcycle(ptq2_mg *mg, const double *r, double *z) {
   if (!mg || !r || !z) return PTQ2_ERR_INVALID_ARG;
-  const int64_t n = mg->n[0];
-  MGCHK(hipMemcpyAsync(mg->r_in, r, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
-  if (mg->exec) MGCHK(hipGraphLaunch(mg->exec, mg->stream));
-  else MGRC(vcycle_body(mg));
-  MGCHK(hipMemcpyAsync(z, mg->z_out, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
-  return PTQ2_OK;
+  return vcycle_scaled(mg, r, 1.0, z);
 }
 
 int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_it, int restart, int *iters,
@@ -982,10 +1193,36 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
     return PTQ2_OK;
   }
   const double target = rtol * bnorm;
-  std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), h(restart + 1),
-      h2(restart + 1), y(restart);
+  // Arnoldi with delayed reorthogonalisation.  Slot j of V holds the candidate u_j unnormalised
+  // (u_j = su * slot j) until step j finishes it: one pass computes its reorthogonalisation
+  // coefficients a = V_{j-1}^T u_j together with the projection b = V_{j-1}^T w of w = A z_j,
+  // and a second pass writes v_j = (u_j - V a) / alpha over the slot and the next candidate
+  // w - V b - h_jj v_j into slot j+1.  Column j-1 of H becomes final in step j (the
+  // reorthogonalisation moves H(j, j-1) u_j into V a + alpha v_j), so the residual estimate
+  // trails the basis by one step.  z_j = M^-1 u_j: FGMRES only needs A Z = V H for the stored
+  // Z, so the preconditioner may see the candidate before its reorthogonalisation.
+  std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1), y(restart),
+      dots(2 * (size_t)restart + 4), ab(2 * (size_t)restart + 2), av(restart + 1);
+  auto Hc = [&](int i, int j) -> double & { return H[(size_t)i * restart + j]; };
+  // rotate the finished column j, extend the rotations, return the residual estimate
+  auto givens = [&](int j) {
+    for (int i = 0; i < j; i++) {
+      const double t = cs[i] * Hc(i, j) + sn[i] * Hc(i + 1, j);
+      Hc(i + 1, j) = -sn[i] * Hc(i, j) + cs[i] * Hc(i + 1, j);
+      Hc(i, j) = t;
+    }
+    const double den = std::hypot(Hc(j, j), Hc(j + 1, j));
+    cs[j] = den > 0 ? Hc(j, j) / den : 1.0;
+    sn[j] = den > 0 ? Hc(j + 1, j) / den : 0.0;
+    Hc(j, j) = den;
+    Hc(j + 1, j) = 0.0;
+    g[j + 1] = -sn[j] * g[j];
+    g[j] = cs[j] * g[j];
+    return std::fabs(g[j + 1]);
+  };
+  const double **Vp = mg->Vptr;
   while (it < max_it) {
-    // r = b - A x
+    // r = b - A x, the unnormalised first candidate
     MGRC(ptq2_a11_apply(fc, x, mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
     hipLaunchKernelGGL(k_sub, dim3(gb), dim3(TPB), 0, mg->stream, n, b, (const double *)mg->w, mg->Vk[0]);
     double beta;
@@ -994,65 +1231,113 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
       rn = beta;
       break;
     }
-    hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->Vk[0], 1.0 / beta, mg->Vk[0]);
     std::fill(g.begin(), g.end(), 0.0);
+    std::fill(H.begin(), H.end(), 0.0);
     g[0] = beta;
-    int j = 0;
-    for (; j < restart && it < max_it; j++) {
-      MGRC(ptq2_mg_vcycle(mg, mg->Vk[j], mg->Zk[j]));
+    double su = 1.0 / beta;  // u_j = su * slot j
+    double bprev = 0.0;      // H(j, j-1) before the reorthogonalisation of u_j
+    int k = 0;               // finished columns of this cycle
+    for (int j = 0; j < restart && it < max_it; j++) {
+      MGRC(vcycle_scaled(mg, mg->Vk[j], su, mg->Zk[j]));
       MGRC(ptq2_a11_apply(fc, mg->Zk[j], mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
-      // classical Gram-Schmidt; a second pass only when the first one cancelled more than
-      // half of w's norm (the DGKS criterion: ||w'|| < ||w|| / sqrt(2))
-      double wn;
-      MGRC(norm2(mg, mg->w, n, &wn));
-      MGRC(mdot(mg, mg->Vptr, j + 1, mg->w, n, h.data()));
-      MGCHK(hipMemcpyAsync(mg->hdev, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
-      hipLaunchKernelGGL(k_maxpy, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Vptr, j + 1, (const double *)mg->hdev, mg->w);
-      double hn;
-      MGRC(norm2(mg, mg->w, n, &hn));
-      if (hn < 0.7071067811865476 * wn) {
-        MGRC(mdot(mg, mg->Vptr, j + 1, mg->w, n, h2.data()));
-        MGCHK(hipMemcpyAsync(mg->hdev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
-        hipLaunchKernelGGL(k_maxpy, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Vptr, j + 1, (const double *)mg->hdev,
-                           mg->w);
-        for (int i = 0; i <= j; i++) h[i] += h2[i];
-        MGRC(norm2(mg, mg->w, n, &hn));
-      }
-      h[j + 1] = hn;
-      if (hn > 0.0)
-        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->w, 1.0 / hn, mg->Vk[j + 1]);
-      // Givens rotations
-      for (int i = 0; i < j; i++) {
-        double t = cs[i] * h[i] + sn[i] * h[i + 1];
-        h[i + 1] = -sn[i] * h[i] + cs[i] * h[i + 1];
-        h[i] = t;
-      }
-      double den = std::hypot(h[j], h[j + 1]);
-      cs[j] = den > 0 ? h[j] / den : 1.0;
-      sn[j] = den > 0 ? h[j + 1] / den : 0.0;
-      h[j] = den;
-      h[j + 1] = 0.0;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      for (int i = 0; i <= j; i++) H[(size_t)i * restart + j] = h[i];
       it++;
-      rn = std::fabs(g[j + 1]);
-      mg->hist.push_back(rn);
-      if (rn <= target || hn == 0.0) {
-        j++;
+      MGRC(mdot2(mg, j, mg->Vk[j], mg->w, n, dots.data()));
+      double aa = 0.0, abd = 0.0;
+      for (int v = 0; v < j; v++) {
+        av[v] = su * dots[2 * v];
+        aa += av[v] * av[v];
+        abd += av[v] * dots[2 * v + 1];
+      }
+      const double c = su * dots[2 * j], d = su * su * dots[2 * j + 1];
+      const double alpha2 = d - aa;
+      double alpha, hjj;
+      if (alpha2 > 0.5 * d) {
+        alpha = std::sqrt(alpha2);
+        hjj = (c - abd) / alpha;
+      } else {
+        // u_j lost more than half its norm to V: finish it explicitly and project w by
+        // plain CGS2 against the finished basis
+        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->Vk[j], su,
+                           mg->Vk[j]);
+        if (j > 0) MGCHK(hipMemcpyAsync(mg->hdev, av.data(), sizeof(double) * j, hipMemcpyHostToDevice, mg->stream));
+        MGRC(maxpy_norm(mg, j, mg->Vk[j], n, &alpha));
+        if (!(alpha > 0.0)) return mgfail(mg, PTQ2_ERR_DEVICE, "FGMRES: Krylov candidate vanished");
+        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->Vk[j], 1.0 / alpha,
+                           mg->Vk[j]);
+        hjj = std::nan("");
+      }
+      // column j-1 is now final
+      if (j > 0) {
+        for (int i = 0; i < j; i++) Hc(i, j - 1) += bprev * av[i];
+        Hc(j, j - 1) = bprev * alpha;
+        rn = givens(j - 1);
+        mg->hist.push_back(rn);
+        k = j;
+        if (rn <= target) break;  // z_j and w are not needed
+      }
+      double bn;
+      if (!std::isnan(hjj)) {
+        const double gam = hjj / alpha;
+        for (int v = 0; v < j; v++) {
+          Hc(v, j) = dots[2 * v + 1];
+          ab[v] = av[v];
+          ab[j + v] = dots[2 * v + 1] - gam * av[v];
+        }
+        Hc(j, j) = hjj;
+        if (j > 0) MGCHK(hipMemcpyAsync(mg->hdev, ab.data(), sizeof(double) * 2 * j, hipMemcpyHostToDevice, mg->stream));
+        MGRC(dcgs_update(mg, j, su, 1.0 / alpha, gam, mg->Vk[j], mg->Vk[j + 1], n, &bn));
+      } else {
+        // CGS with the DGKS second pass against v_0..v_j
+        std::vector<double> h(j + 2), h2(j + 2);
+        MGRC(mdot(mg, Vp, j + 1, mg->w, n, true, h.data()));
+        const double wn = std::sqrt(std::max(h[j + 1], 0.0));
+        MGCHK(hipMemcpyAsync(mg->hdev, h.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
+        MGRC(maxpy_norm(mg, j + 1, mg->w, n, &bn));
+        if (bn < 0.7071067811865476 * wn) {
+          MGRC(mdot(mg, Vp, j + 1, mg->w, n, false, h2.data()));
+          MGCHK(hipMemcpyAsync(mg->hdev, h2.data(), sizeof(double) * (j + 1), hipMemcpyHostToDevice, mg->stream));
+          MGRC(maxpy_norm(mg, j + 1, mg->w, n, &bn));
+          for (int i = 0; i <= j; i++) h[i] += h2[i];
+        }
+        for (int i = 0; i <= j; i++) Hc(i, j) = h[i];
+        MGCHK(hipMemcpyAsync(mg->Vk[j + 1], mg->w, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+      }
+      Hc(j + 1, j) = bn;
+      bprev = bn;
+      if (bn == 0.0) {  // the Krylov space is invariant: column j is final
+        rn = givens(j);
+        mg->hist.push_back(rn);
+        k = j + 1;
         break;
       }
+      su = 1.0 / bn;
+      if (j == restart - 1 || it >= max_it) {
+        // end of the cycle: finish column j with the reorthogonalisation of u_{j+1}
+        MGRC(mdot(mg, Vp, j + 1, mg->Vk[j + 1], n, true, dots.data()));
+        double a2 = 0.0;
+        for (int v = 0; v <= j; v++) {
+          const double a = su * dots[v];
+          Hc(v, j) += bprev * a;
+          a2 += a * a;
+        }
+        Hc(j + 1, j) = bprev * std::sqrt(std::max(su * su * dots[j + 1] - a2, 0.0));
+        rn = givens(j);
+        mg->hist.push_back(rn);
+        k = j + 1;
+      }
     }
-    // x += Z y, with H(0:j,0:j) y = g
-    for (int i = j - 1; i >= 0; i--) {
+    // x += Z y, with H(0:k,0:k) y = g
+    for (int i = k - 1; i >= 0; i--) {
       double s = g[i];
-      for (int k = i + 1; k < j; k++) s -= H[(size_t)i * restart + k] * y[k];
-      y[i] = s / H[(size_t)i * restart + i];
+      for (int q = i + 1; q < k; q++) s -= Hc(i, q) * y[q];
+      y[i] = s / Hc(i, i);
     }
-    MGCHK(hipMemcpyAsync(mg->hdev, y.data(), sizeof(double) * j, hipMemcpyHostToDevice, mg->stream));
-    hipLaunchKernelGGL(k_mcombine, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Zptr, j, (const double *)mg->hdev, x);
-    // converged by the recurrence estimate: the next pass computes the true residual and
-    // stops there if it meets the target, else restarts from x
+    if (k > 0) {
+      MGCHK(hipMemcpyAsync(mg->hdev, y.data(), sizeof(double) * k, hipMemcpyHostToDevice, mg->stream));
+      hipLaunchKernelGGL(k_mcombine, dim3(gb), dim3(TPB), 0, mg->stream, n, mg->Zptr, k, (const double *)mg->hdev, x);
+    }
+    // the next pass computes the true residual and stops there if it meets the target,
+    // else restarts from x
   }
   // true residual
   MGRC(ptq2_a11_apply(fc, x, mg->w, PTQ2_DEVICE | PTQ2_MATMULT));

@@ -16,7 +16,10 @@ from ._lib import check
 class MGOpts(ctypes.Structure):
     _fields_ = [("nlevels", ctypes.c_int), ("smooth_its", ctypes.c_int), ("fp32_smoother", ctypes.c_int),
                 ("emin_frac", ctypes.c_double), ("emax_frac", ctypes.c_double), ("esteig_its", ctypes.c_int),
-                ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int), ("galerkin_coarse", ctypes.c_int)]
+                ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int), ("galerkin_coarse", ctypes.c_int),
+                ("coarse_eta_mean", ctypes.c_int)]
+
+ETA_MEANS = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "max": 3}
 
 
 _SOLVER_SIGS = {
@@ -57,12 +60,19 @@ class ViscousSolver:
     """GMG V-cycle (hipGraph) + FGMRES for MatMult_MFStokes_A11 x = b on one GPU."""
 
     def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000,
-                 galerkin_coarse=True):
+                 galerkin_coarse=True, eta_mean="arithmetic", emin_frac=None, emax_frac=None):
         L = _sl()
         o = default_opts()
         o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
         o.esteig_its, o.use_graph, o.coarse_max_dofs = esteig_its, 1 if use_graph else 0, coarse_max_dofs
         o.galerkin_coarse = 1 if galerkin_coarse else 0
+        if eta_mean not in ETA_MEANS:
+            raise ValueError("eta_mean must be one of %s" % sorted(ETA_MEANS))
+        o.coarse_eta_mean = ETA_MEANS[eta_mean]
+        if emin_frac is not None:
+            o.emin_frac = emin_frac
+        if emax_frac is not None:
+            o.emax_frac = emax_frac
         self.ctx = ctx  # keep the fine context alive
         h = ctypes.c_void_p()
         check(L.ptq2_mg_create(ctypes.byref(h), ctx.handle, ctypes.byref(o)), "ptq2_mg_create", ctx.handle)
