@@ -114,7 +114,6 @@ int ptq2_internal_build_fac32(ptq2_ctx *c, bool block) {
 
 void *ptq2_internal_tilebuf(ptq2_ctx *c, int *rc) {
   *rc = PTQ2_OK;
-  if (c->nbricks >= 8 * (int64_t)c->simd_slots) return nullptr;
   if (!c->tilebuf) *rc = dev_alloc(c, &c->tilebuf, sizeof(double) * (size_t)c->nbricks * ptq2::TILEVALS);
   return c->tilebuf;
 }
@@ -193,10 +192,6 @@ int ptq2_create(ptq2_ctx **out, int device) {
     return PTQ2_ERR_DEVICE;
   }
   c->stream = c->own_stream;
-  int cus = 0;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
-    c->simd_slots = 4 * cus;
-  (void)hipGetLastError();
   if (hipMalloc((void **)&c->err, sizeof(int)) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipStreamDestroy(c->own_stream);

@@ -28,11 +28,10 @@ struct ptq2_ctx {
   bool has_state = false;
   uint64_t state = 0;
   int *err = nullptr;
-  // output path of the A11 applies: meshes whose bricks cannot fill 8 coloured launches
-  // (fewer than 8 x the device's wave slots) run all bricks in one launch and sum the
-  // tiles with a gather pass (ptq2::launch_a11_*'s tilebuf); tilebuf holds TILEVALS
-  // doubles per brick
-  int simd_slots = 1024;
+  // output path of the A11 applies: all bricks in one launch, each writing its output tile
+  // to its slot of tilebuf (TILEVALS doubles per brick), then a gather pass sums the slots
+  // (ptq2::launch_a11_*).  One launch keeps the memory phases of some bricks overlapped
+  // with the arithmetic of others, which 8 coloured launches of a few waves per SIMD do not.
   void *tilebuf = nullptr;
   // staging for host-pointer calls
   void *su = nullptr, *sy = nullptr;

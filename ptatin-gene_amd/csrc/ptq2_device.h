@@ -89,6 +89,15 @@ __device__ __forceinline__ BrickId brick_lin(const Box &b, int bid) {
   return r;
 }
 
+// brick of an uncoloured launch, XCD-aware: workgroups are dealt to the 8 XCDs in turn
+// (workgroup w runs on XCD w % 8), and XCD k takes the k-th contiguous eighth of the bricks
+// in x-fastest order, so bricks that share tile faces mostly run on one L2
+__device__ __forceinline__ BrickId brick_xcd(const Box &b, int bid) {
+  const int nb = b.nbx * b.nby * b.nbz;
+  const int q = nb >> 3, r = nb & 7, k = bid & 7, i = bid >> 3;
+  return brick_lin(b, k * q + min(k, r) + i);
+}
+
 struct Tile {
   int ex0, ey0, ez0;   // first element of the brick
   int nex, ney, nez;   // elements in the brick per direction

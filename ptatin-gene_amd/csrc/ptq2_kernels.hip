@@ -83,7 +83,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
   __shared__ double Xs[3 * TCS];
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
-  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_lin(b, blockIdx.x);
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
   double en[3];
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   float *Fs = Us + 3 * TCS;
   static_assert(sizeof(float) * (3 * TCS + FROW) <= sizeof(Ts), "velocity tile + factor row fit the output tile");
   const int lane = threadIdx.x;
-  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_lin(b, blockIdx.x);
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const float *fsrc = fac + (int64_t)br.lin * 243 * BEL;
   lds_copy_async<FROW * 4>(fsrc, Fs, lane);
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64g(Box b, int color, const doubl
                                                    int add_mode, double *__restrict__ tilebuf) {
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
-  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_lin(b, blockIdx.x);
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const double *gbase = geo + (int64_t)br.lin * 243 * BEL + lane;
   const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
@@ -896,42 +896,92 @@ static inline int nbricks_color(const Box &b, int color) {
 static inline int nbricks(const Box &b) { return b.nbx * b.nby * b.nbz; }
 
 // Sum of the tile-buffer slots of every covered node, bricks in increasing order
-// (deterministic): the uncoloured output path of the A11 kernels.
+// (deterministic): the uncoloured output path of the A11 kernels.  One workgroup per brick
+// sums the nodes the brick owns (local 0..7 per axis, and the upper boundary nodes for the
+// last brick of an axis): its own slot plus, on its lower faces, the upper faces, edges and
+// corner of the up to 7 lower neighbours (which the XCD-aware brick order keeps on the same
+// L2).  Everything is first staged in LDS by one batch of independent loads (the own slot is
+// one contiguous run), then summed; the writes are row-contiguous in y.
 template <typename T>
-__global__ void k_gather(Box b, const T *__restrict__ buf, T *__restrict__ y, int add_mode) {
-  const int64_t nx = 2 * (int64_t)b.mx + 1, ny = 2 * (int64_t)b.my + 1, nz = 2 * (int64_t)b.mz + 1;
-  const int64_t nn = nx * ny * nz;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nn; t += (int64_t)gridDim.x * blockDim.x) {
-    const int li = (int)(t % nx), lj = (int)((t / nx) % ny), lk = (int)(t / (nx * ny));
-    // (brick, local node) candidates per axis: the brick holding the node, and the previous
-    // brick when the node is on its upper face
-    int bxs[2], ixs[2], nxc = 0, bys[2], iys[2], nyc = 0, bzs[2], izs[2], nzc = 0;
-    if (li % (2 * BE) == 0 && li > 0) { bxs[nxc] = li / (2 * BE) - 1; ixs[nxc++] = 2 * BE; }
-    if (li / (2 * BE) < b.nbx) { bxs[nxc] = li / (2 * BE); ixs[nxc++] = li % (2 * BE); }
-    if (lj % (2 * BE) == 0 && lj > 0) { bys[nyc] = lj / (2 * BE) - 1; iys[nyc++] = 2 * BE; }
-    if (lj / (2 * BE) < b.nby) { bys[nyc] = lj / (2 * BE); iys[nyc++] = lj % (2 * BE); }
-    if (lk % (2 * BE) == 0 && lk > 0) { bzs[nzc] = lk / (2 * BE) - 1; izs[nzc++] = 2 * BE; }
-    if (lk / (2 * BE) < b.nbz) { bzs[nzc] = lk / (2 * BE); izs[nzc++] = lk % (2 * BE); }
-    T acc[3] = {T(0), T(0), T(0)};
-    for (int cz = 0; cz < nzc; cz++)
-      for (int cy = 0; cy < nyc; cy++)
-        for (int cx = 0; cx < nxc; cx++) {
-          const int64_t lin = bxs[cx] + (int64_t)b.nbx * (bys[cy] + (int64_t)b.nby * bzs[cz]);
-          const T *o = buf + lin * TILEVALS + (izs[cz] * TN + iys[cy]) * TN + ixs[cx];
+__global__ __launch_bounds__(256) void k_gather(Box b, const T *__restrict__ buf, T *__restrict__ y, int add_mode) {
+  constexpr int TV3 = TN * TN * TN, F2 = TN * TN, UP = 2 * BE;
+  // per component: own slot (TV3), faces X [iz][iy], Y [iz][ix], Z [iy][ix] of the lower
+  // neighbours, edges XY [iz], XZ [iy], YZ [ix], corner
+  constexpr int OFX = 3 * TV3, OFY = OFX + 3 * F2, OFZ = OFY + 3 * F2, OXY = OFZ + 3 * F2, OXZ = OXY + 3 * TN,
+                OYZ = OXZ + 3 * TN, OC = OYZ + 3 * TN, NS = OC + 3;
+  __shared__ T sh[NS];
+  const BrickId br = brick_xcd(b, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  const int64_t lin = br.lin, sx = TILEVALS, sy = (int64_t)b.nbx * TILEVALS, sz = (int64_t)b.nbx * b.nby * TILEVALS;
+  const bool hx = br.bx > 0, hy = br.by > 0, hz = br.bz > 0;
+  const T *own = buf + lin * TILEVALS;
+  // every load is issued before the first LDS store (one exposed latency per block); the
+  // neighbour addresses fall back to the own slot where a neighbour does not exist
+  constexpr int NV = (3 * TV3 + 255) / 256;
+  const int tid = threadIdx.x;
+  T v[NV];
 #pragma unroll
-          for (int c = 0; c < 3; c++) acc[c] += o[c * TN * TN * TN];
-        }
-    const int64_t gi = 3 * ((int64_t)(b.i0 + li) + (int64_t)b.NX * ((int64_t)(b.j0 + lj) + (int64_t)b.NY * (b.k0 + lk)));
+  for (int m = 0; m < NV; m++) v[m] = own[min(tid + 256 * m, 3 * TV3 - 1)];
+  const int g = min(tid, 3 * F2 - 1);
+  const int gc = g / F2, gr = g - gc * F2, gq = gr / TN, gp = gr - gq * TN;  // gr = gq*TN + gp
+  const T fx = own[(hx ? -sx : 0) + gc * TV3 + gr * TN + UP];              // (UP, iy=gp, iz=gq)
+  const T fy = own[(hy ? -sy : 0) + gc * TV3 + (gq * TN + UP) * TN + gp];  // (ix=gp, UP, iz=gq)
+  const T fz = own[(hz ? -sz : 0) + gc * TV3 + (UP * TN + gq) * TN + gp];  // (ix=gp, iy=gq, UP)
+  const int e = min(tid, 3 * TN - 1), ec = e / TN, ep = e - ec * TN;
+  const T exy = own[(hx && hy ? -sx - sy : 0) + ec * TV3 + (ep * TN + UP) * TN + UP];  // (UP, UP, iz=ep)
+  const T exz = own[(hx && hz ? -sx - sz : 0) + ec * TV3 + (UP * TN + ep) * TN + UP];  // (UP, iy=ep, UP)
+  const T eyz = own[(hy && hz ? -sy - sz : 0) + ec * TV3 + (UP * TN + UP) * TN + ep];  // (ix=ep, UP, UP)
+  const int cc = min(tid, 2);
+  const T cxyz = own[(hx && hy && hz ? -sx - sy - sz : 0) + cc * TV3 + (UP * TN + UP) * TN + UP];
 #pragma unroll
-    for (int c = 0; c < 3; c++) y[gi + c] = add_mode ? y[gi + c] + acc[c] : acc[c];
+  for (int m = 0; m < NV; m++)
+    if (tid + 256 * m < 3 * TV3) sh[tid + 256 * m] = v[m];
+  if (tid < 3 * F2) {
+    sh[OFX + tid] = fx;
+    sh[OFY + tid] = fy;
+    sh[OFZ + tid] = fz;
+  }
+  if (tid < 3 * TN) {
+    sh[OXY + tid] = exy;
+    sh[OXZ + tid] = exz;
+    sh[OYZ + tid] = eyz;
+  }
+  if (tid < 3) sh[OC + tid] = cxyz;
+  __syncthreads();
+  const int ox = br.bx == b.nbx - 1 ? t.tnx : UP;
+  const int oy = br.by == b.nby - 1 ? t.tny : UP;
+  const int oz = br.bz == b.nbz - 1 ? t.tnz : UP;
+  // thread = (row group rg, interlaced column k = 3 ix + c); rows (iy, iz) step by TN
+  const int rg = threadIdx.x / RLEN, k = threadIdx.x - RLEN * rg;
+  const int ix = k / 3, c = k - 3 * ix;
+  if (rg >= TN || ix >= ox) return;
+  const bool nx = hx && ix == 0;
+  T *yb = y + 3 * t.node0 + k;
+  const int64_t ys = 3 * (int64_t)b.NX, zs = 3 * (int64_t)b.NX * b.NY;
+  int iy = rg, iz = 0;
+  while (iy >= oy) { iy -= oy; iz++; }
+  while (iz < oz) {
+    const bool ny = hy && iy == 0, nz = hz && iz == 0;
+    // lower neighbours in increasing brick index, the own slot last
+    T acc = T(0);
+    if (nz && ny && nx) acc += sh[OC + c];
+    if (nz && ny) acc += sh[OYZ + c * TN + ix];
+    if (nz && nx) acc += sh[OXZ + c * TN + iy];
+    if (nz) acc += sh[OFZ + c * F2 + iy * TN + ix];
+    if (ny && nx) acc += sh[OXY + c * TN + iz];
+    if (ny) acc += sh[OFY + c * F2 + iz * TN + ix];
+    if (nx) acc += sh[OFX + c * F2 + iz * TN + iy];
+    acc += sh[c * TV3 + (iz * TN + iy) * TN + ix];
+    T *py = yb + iy * ys + iz * zs;
+    *py = add_mode ? *py + acc : acc;
+    iy += TN;
+    while (iy >= oy) { iy -= oy; iz++; }
   }
 }
 
 template <typename T>
 hipError_t launch_gather(const Box &b, const T *buf, T *y, bool add, hipStream_t s) {
-  const int64_t nn = (2 * (int64_t)b.mx + 1) * (2 * (int64_t)b.my + 1) * (2 * (int64_t)b.mz + 1);
-  const int blocks = (int)std::min<int64_t>((nn + 255) / 256, 65536);
-  hipLaunchKernelGGL(k_gather<T>, dim3(blocks), dim3(256), 0, s, b, buf, y, add ? 1 : 0);
+  hipLaunchKernelGGL(k_gather<T>, dim3(nbricks(b)), dim3(256), 0, s, b, buf, y, add ? 1 : 0);
   return hipGetLastError();
 }
 
