@@ -47,10 +47,11 @@ enum {
                              Dirichlet rows replaced on output */
 #define PTQ2_ADD 0x8u     /* y += (element sum), the SpMVOp raw-kernel contract;
                              without it y = (element sum) */
-#define PTQ2_ONTHEFLY 0x10u /* FP64 A11: recompute the element geometry from the
-                               coordinates in every apply instead of reading the resident
-                               per-point factors (built on the first FP64 apply after a mesh
-                               set-up, 1944 B per element) */
+#define PTQ2_ONTHEFLY 0x10u /* FP64 A11: recompute the element geometry from the coordinates
+                               in every apply (the default; the flag only makes it explicit) */
+#define PTQ2_STOREDGEO 0x20u /* FP64 A11: read resident per-point geometric factors instead
+                               (1944 B per element, built on the first such apply after a
+                               mesh set-up); slower than recomputing on MI355X */
 
 /* lifecycle ------------------------------------------------------------- */
 /* replaces MFA11CUDA_SetUp (include/stokes_operators.h:124-149) and the

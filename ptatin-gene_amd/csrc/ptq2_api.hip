@@ -112,10 +112,10 @@ int ptq2_internal_build_fac32(ptq2_ctx *c, bool block) {
   return PTQ2_OK;
 }
 
-void *ptq2_internal_tilebuf(ptq2_ctx *c, int *rc) {
+void *ptq2_internal_facebuf(ptq2_ctx *c, int *rc) {
   *rc = PTQ2_OK;
-  if (!c->tilebuf) *rc = dev_alloc(c, &c->tilebuf, sizeof(double) * (size_t)c->nbricks * ptq2::TILEVALS);
-  return c->tilebuf;
+  if (!c->facebuf) *rc = dev_alloc(c, &c->facebuf, sizeof(double) * (size_t)c->nbricks * ptq2::FACEVALS);
+  return c->facebuf;
 }
 
 namespace {
@@ -208,7 +208,7 @@ int ptq2_destroy(ptq2_ctx **pc) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
-  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->tilebuf);
+  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->facebuf);
   dev_free(c->err); dev_free(c->su); dev_free(c->sy); dev_free(c->sp); dev_free(c->syp);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
@@ -262,9 +262,9 @@ int ptq2_setup_mesh_box(ptq2_ctx *c, int mx, int my, int mz, int NX, int NY, int
   // release a previous mesh
   HIPCHK(c, hipStreamSynchronize(c->stream));
   dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
-  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->tilebuf);
+  dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->facebuf);
   c->coords = nullptr; c->mask = nullptr; c->dir_idx = nullptr; c->eta_bm = nullptr;
-  c->fac32 = nullptr; c->facb32 = nullptr; c->geo64 = nullptr; c->tilebuf = nullptr;
+  c->fac32 = nullptr; c->facb32 = nullptr; c->geo64 = nullptr; c->facebuf = nullptr;
   c->mesh_ready = c->eta_ready = c->fac32_valid = c->facb32_valid = c->geo64_valid = c->has_state = false;
   c->ndir = 0;
 
@@ -367,12 +367,14 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
   int rc = check_ready(c, true);
   if (rc) return rc;
   if (!u || !y) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
-  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY))
+  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY | PTQ2_STOREDGEO))
     return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
   if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
+  if ((flags & PTQ2_ONTHEFLY) && (flags & PTQ2_STOREDGEO))
+    return fail(c, PTQ2_ERR_INVALID_ARG, "ONTHEFLY and STOREDGEO exclusive");
   if ((rc = set_device(c))) return rc;
   const bool dev = flags & PTQ2_DEVICE, f32 = flags & PTQ2_FP32, mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
-  const bool otf = flags & PTQ2_ONTHEFLY;
+  const bool otf = !(flags & PTQ2_STOREDGEO);
   const uint8_t *mask = mm ? c->mask : nullptr;
   const int64_t n = c->nvel;
   const size_t es = f32 ? sizeof(float) : sizeof(double);
@@ -389,7 +391,7 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
     du = c->su;
     dy = c->sy;
   }
-  void *tb = ptq2_internal_tilebuf(c, &rc);
+  void *tb = ptq2_internal_facebuf(c, &rc);
   if (rc) return rc;
   if (f32) {
     HIPCHK(c, ptq2::launch_a11_fp32(c->box, (const float *)du, mask, c->fac32, (float *)dy, add, c->stream,
@@ -498,12 +500,14 @@ int ptq2_stokes_apply(ptq2_ctx *c, const void *u, const void *p, void *yu, void 
   int rc = check_ready(c, true);
   if (rc) return rc;
   if (!u || !p || !yu || !yp) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
-  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY))
+  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY | PTQ2_STOREDGEO))
     return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
   if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
+  if ((flags & PTQ2_ONTHEFLY) && (flags & PTQ2_STOREDGEO))
+    return fail(c, PTQ2_ERR_INVALID_ARG, "ONTHEFLY and STOREDGEO exclusive");
   if ((rc = set_device(c))) return rc;
   const bool dev = flags & PTQ2_DEVICE, f32 = flags & PTQ2_FP32, mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
-  const bool otf = flags & PTQ2_ONTHEFLY;
+  const bool otf = !(flags & PTQ2_STOREDGEO);
   const uint8_t *mask = mm ? c->mask : nullptr;
   const size_t es = f32 ? sizeof(float) : sizeof(double);
   if (f32 && (rc = ptq2_internal_build_fac32(c, true))) return rc;
@@ -527,19 +531,21 @@ int ptq2_stokes_apply(ptq2_ctx *c, const void *u, const void *p, void *yu, void 
     }
     du = c->su; dp = c->sp; dyu = c->sy; dyp = c->syp;
   }
+  void *fb = ptq2_internal_facebuf(c, &rc);
+  if (rc) return rc;
   if (f32) {
     // one fused pass (MFStokesWrapper_A_AVX's single element loop), FP32 factors
     HIPCHK(c, ptq2::launch_stokes_fp32(c->box, (const float *)du, (const float *)dp, mask, c->fac32, c->facb32,
-                                       (float *)dyu, (float *)dyp, add, c->stream));
+                                       (float *)dyu, (float *)dyp, add, c->stream, (float *)fb));
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, (const float *)du, (float *)dyu, 0, c->stream));
   } else {
     // momentum A11 u + A12 p, continuity A21 u (MFStokesWrapper_A_AVX in three passes)
     if (otf)
       HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dyu, add,
-                                      c->stream));
+                                      c->stream, (double *)fb));
     else
       HIPCHK(c, ptq2::launch_a11_fp64g(c->box, (const double *)du, mask, c->geo64, c->eta_bm, (double *)dyu, add,
-                                       c->stream));
+                                       c->stream, (double *)fb));
     HIPCHK(c, ptq2::launch_a12(c->box, c->coords, (const double *)dp, (double *)dyu, true, c->stream));
     HIPCHK(c, ptq2::launch_a21(c->box, c->coords, (const double *)du, mask, (double *)dyp, add, c->stream));
     if (mm)

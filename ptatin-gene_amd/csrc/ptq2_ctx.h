@@ -28,11 +28,12 @@ struct ptq2_ctx {
   bool has_state = false;
   uint64_t state = 0;
   int *err = nullptr;
-  // output path of the A11 applies: all bricks in one launch, each writing its output tile
-  // to its slot of tilebuf (TILEVALS doubles per brick), then a gather pass sums the slots
-  // (ptq2::launch_a11_*).  One launch keeps the memory phases of some bricks overlapped
-  // with the arithmetic of others, which 8 coloured launches of a few waves per SIMD do not.
-  void *tilebuf = nullptr;
+  // output path of the A11 applies: all bricks in one launch, each storing the nodes it owns
+  // and its upper-face values into its slot of facebuf (FACEVALS doubles per brick), then a
+  // pass adds the face slots into their owners (ptq2::launch_a11_*).  One launch keeps the
+  // memory phases of some bricks overlapped with the arithmetic of others, which 8 coloured
+  // launches of a few waves per SIMD do not.
+  void *facebuf = nullptr;
   // staging for host-pointer calls
   void *su = nullptr, *sy = nullptr;
   size_t su_bytes = 0, sy_bytes = 0;
@@ -47,4 +48,4 @@ struct ptq2_ctx {
 int ptq2_internal_build_fac32(ptq2_ctx *c, bool block = false);
 // the tile buffer of the uncoloured output path if this mesh uses it (allocated on first
 // use), else nullptr; *rc receives an allocation error
-void *ptq2_internal_tilebuf(ptq2_ctx *c, int *rc);
+void *ptq2_internal_facebuf(ptq2_ctx *c, int *rc);

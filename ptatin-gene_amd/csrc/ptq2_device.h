@@ -272,17 +272,48 @@ __device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, cons
   }
 }
 
-// Uncoloured output: the whole LDS output tile of brick br to its slot of a tile
-// buffer, dense [c][z][y][x] with 9 nodes per edge (k_gather sums the slots).
+// Uncoloured output, owner writes: every tile node has one owning brick, the brick whose
+// local coordinates of the node are all below 2*BE (or, for the last brick of an axis, that
+// axis' upper boundary node).  The brick stores (or, in add mode, adds) the nodes it owns
+// straight into g, and the nodes of its upper faces that a neighbour owns into its face slot
+// of FACEVALS values: three TN x TN planes of interlaced 3-vectors, X [z][y] (x = 2*BE),
+// Y [z][x] (y = 2*BE), Z [y][x] (z = 2*BE), a node going to the first of X, Y, Z that
+// applies.  k_gather_faces adds the slots into their owners.
 constexpr int TILEVALS = 3 * TN * TN * TN;
+constexpr int FACEVALS = 3 * 3 * TN * TN;
 template <typename S, typename T>
-__device__ __forceinline__ void store_tile_buf(const BrickId &br, const S *__restrict__ s, T *__restrict__ buf,
-                                               int lane) {
-  T *o = buf + (int64_t)br.lin * TILEVALS;
-  for (int f = lane; f < TILEVALS; f += BEL) {
-    const int c = f / (TN * TN * TN), r = f - c * (TN * TN * TN);
-    const int z = r / (TN * TN), y = (r / TN) % TN, x = r % TN;
-    o[f] = T(s[c * TCS + z * TPS + y * TRS + tx(x)]);
+__device__ __forceinline__ void store_tile_own(const Box &b, const BrickId &br, const Tile &t, const S *__restrict__ s,
+                                               T *__restrict__ g, T *__restrict__ face, bool add_mode, int lane) {
+  asm volatile("" : "+v"(lane));  // see store_tile
+  const TileLane L = tile_lane(b, t, br, lane);
+  const int col = lane - RLEN * (L.p < 2 ? L.p : 0);
+  const int ii = col / 3, c = col - 3 * ii;
+  const bool ownx = ii < 2 * BE || br.bx == b.nbx - 1;
+  const bool lasty = br.by == b.nby - 1, lastz = br.bz == b.nbz - 1;
+  T *fo = face + (int64_t)br.lin * FACEVALS + c;
+  T v[TPAIRS];
+  if (add_mode) {
+#pragma unroll
+    for (int it = 0; it < TPAIRS; it++) {
+      int y, z;
+      tile_row(it, L, y, z);
+      v[it] = g[tile_gidx(t, L, y, z)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    int y, z;
+    tile_row(it, L, y, z);
+    if (!(L.col_ok && y < t.tny && z < t.tnz)) continue;
+    const bool owny = y < 2 * BE || lasty, ownz = z < 2 * BE || lastz;
+    const S val = s[L.lds_col + TRS * y + TPS * z];
+    if (ownx && owny && ownz) {
+      g[tile_gidx(t, L, y, z)] = add_mode ? T(v[it] + val) : T(val);
+    } else {
+      const int slot = !ownx ? z * TN + y : (!owny ? TN * TN + z * TN + ii : 2 * TN * TN + y * TN + ii);
+      fo[3 * slot] = T(val);
+    }
   }
 }
 

@@ -11,19 +11,21 @@ constexpr int ERRBIT_MESH = 1;
 constexpr int ERRBIT_VISCOSITY = 2;
 constexpr int ERRBIT_NONFINITE = 4;
 
-// A11 launchers: with tilebuf (TILEVALS values per brick) all bricks run in one launch
-// and k_gather sums the tiles; without it, 8 coloured launches store/add directly.
+// A11 launchers: with facebuf (FACEVALS values per brick) all bricks run in one launch,
+// storing the nodes they own, and k_gather_faces adds the shared face values into their
+// owners; without it, 8 coloured launches store/add directly.
 hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
-                           const double *eta_bm, double *y, bool add, hipStream_t s, double *tilebuf = nullptr);
+                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf = nullptr);
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
-                           hipStream_t s, float *tilebuf = nullptr);
+                           hipStream_t s, float *facebuf = nullptr);
 hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, float *facb,
                               int *err, hipStream_t s);
 hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,
-                            double *y, bool add, hipStream_t s, double *tilebuf = nullptr);
+                            double *y, bool add, hipStream_t s, double *facebuf = nullptr);
 hipError_t launch_build_geo64(const Box &b, const double *coords, double *geo, hipStream_t s);
 hipError_t launch_stokes_fp32(const Box &b, const float *u, const float *p, const uint8_t *mask, const float *fac,
-                              const float *facb, float *yu, float *yp, bool add, hipStream_t s);
+                              const float *facb, float *yu, float *yp, bool add, hipStream_t s,
+                              float *facebuf = nullptr);
 hipError_t launch_eta_to_bm(const Box &b, const double *eta, double *eta_bm, int *err, hipStream_t s);
 hipError_t launch_check_mesh(const Box &b, const double *coords, int *err, hipStream_t s);
 hipError_t launch_a21(const Box &b, const double *coords, const double *u, const uint8_t *mask, double *yp, bool add,

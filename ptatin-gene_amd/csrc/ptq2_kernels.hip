@@ -79,7 +79,7 @@ __device__ __forceinline__ void load3(const double *__restrict__ ep, int row, do
 __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double *__restrict__ coords,
                                                   const double *__restrict__ u, const uint8_t *__restrict__ mask,
                                                   const double *__restrict__ eta_bm, double *__restrict__ y,
-                                                  int add_mode, double *__restrict__ tilebuf) {
+                                                  int add_mode, double *__restrict__ facebuf) {
   __shared__ double Xs[3 * TCS];
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
@@ -120,7 +120,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
   if (color >= 0)
     store_tile(b, br, t, Xs, y, add_mode != 0, lane);
   else
-    store_tile_buf(br, Xs, tilebuf, lane);
+    store_tile_own(b, br, t, Xs, y, facebuf, add_mode != 0, lane);
 }
 
 // ----------------------------------------------------------------- A11 FP32
@@ -181,7 +181,7 @@ __device__ __forceinline__ void slab_z32(const float *Us, int base, const Coef1 
 
 __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
                                                   const uint8_t *__restrict__ mask, const float *__restrict__ fac,
-                                                  float *__restrict__ y, int add_mode, float *__restrict__ tilebuf) {
+                                                  float *__restrict__ y, int add_mode, float *__restrict__ facebuf) {
   // LDS: the FP32 velocity tile and one quadrature row of factors ([qa*9+m][lane]) during
   // the rows, then the output tile in FP64 (LDS float atomics are many times slower than
   // ds_add_f64 on gfx950, and the wider sum costs nothing in the result)
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   if (color >= 0)
     store_tile(b, br, t, Ts, y, add_mode != 0, lane);
   else
-    store_tile_buf(br, Ts, tilebuf, lane);
+    store_tile_own(b, br, t, Ts, y, facebuf, add_mode != 0, lane);
 }
 
 // ------------------------------------------------------ A11 FP64, stored geometry
@@ -293,7 +293,7 @@ __device__ __forceinline__ void a11_qp64g(const double ur0[3][3], const double u
 __global__ __launch_bounds__(BEL) void k_a11_fp64g(Box b, int color, const double *__restrict__ u,
                                                    const uint8_t *__restrict__ mask, const double *__restrict__ geo,
                                                    const double *__restrict__ eta_bm, double *__restrict__ y,
-                                                   int add_mode, double *__restrict__ tilebuf) {
+                                                   int add_mode, double *__restrict__ facebuf) {
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
   const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
@@ -347,7 +347,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64g(Box b, int color, const doubl
   if (color >= 0)
     store_tile(b, br, t, Us, y, add_mode != 0, lane);
   else
-    store_tile_buf(br, Us, tilebuf, lane);
+    store_tile_own(b, br, t, Us, y, facebuf, add_mode != 0, lane);
 }
 
 // G = sqrt(w/det) adj(J) per point (FP64), for k_a11_fp64g
@@ -430,11 +430,12 @@ __device__ __forceinline__ void stokes_qp32(const float ur0[3][3], const float u
 __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const float *__restrict__ u,
                                                      const float *__restrict__ p, const uint8_t *__restrict__ mask,
                                                      const float *__restrict__ fac, const float *__restrict__ facb,
-                                                     float *__restrict__ yu, float *__restrict__ yp, int add_mode) {
+                                                     float *__restrict__ yu, float *__restrict__ yp, int add_mode,
+                                                     float *__restrict__ facebuf) {
   __shared__ double Ts[3 * TCS];  // FP32 velocity tile during the rows, FP64 output tile after
   float *Us = reinterpret_cast<float *>(Ts);
   const int lane = threadIdx.x;
-  const BrickId br = brick_of(b, color, blockIdx.x);
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
   const float *bbase = facb + (int64_t)br.lin * 108 * BEL + lane;
@@ -493,7 +494,10 @@ __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const flo
   __syncthreads();
   if (active) emit_element(Ts, base, Y);
   __syncthreads();
-  store_tile(b, br, t, Ts, yu, add_mode != 0, lane);
+  if (color >= 0)
+    store_tile(b, br, t, Ts, yu, add_mode != 0, lane);
+  else
+    store_tile_own(b, br, t, Ts, yu, facebuf, add_mode != 0, lane);
 }
 
 // ------------------------------------------------------- pressure basis helper
@@ -895,51 +899,39 @@ static inline int nbricks_color(const Box &b, int color) {
 }
 static inline int nbricks(const Box &b) { return b.nbx * b.nby * b.nbz; }
 
-// Sum of the tile-buffer slots of every covered node, bricks in increasing order
-// (deterministic): the uncoloured output path of the A11 kernels.  One workgroup per brick
-// sums the nodes the brick owns (local 0..7 per axis, and the upper boundary nodes for the
-// last brick of an axis): its own slot plus, on its lower faces, the upper faces, edges and
-// corner of the up to 7 lower neighbours (which the XCD-aware brick order keeps on the same
-// L2).  Everything is first staged in LDS by one batch of independent loads (the own slot is
-// one contiguous run), then summed; the writes are row-contiguous in y.
+// Second pass of the uncoloured output path: every brick adds into the nodes of its lower
+// faces the face slots of its up to 7 lower neighbours (store_tile_own), in increasing brick
+// order after the owner's own value.  The neighbour planes, edges and corner are staged in
+// LDS by one batch of independent loads (the XCD-aware brick order keeps most of them on the
+// same L2 as their writer); then each thread takes up to 3 of the brick's at most 3 x 3 x TN x TN
+// face values.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather(Box b, const T *__restrict__ buf, T *__restrict__ y, int add_mode) {
-  constexpr int TV3 = TN * TN * TN, F2 = TN * TN, UP = 2 * BE;
-  // per component: own slot (TV3), faces X [iz][iy], Y [iz][ix], Z [iy][ix] of the lower
-  // neighbours, edges XY [iz], XZ [iy], YZ [ix], corner
-  constexpr int OFX = 3 * TV3, OFY = OFX + 3 * F2, OFZ = OFY + 3 * F2, OXY = OFZ + 3 * F2, OXZ = OXY + 3 * TN,
-                OYZ = OXZ + 3 * TN, OC = OYZ + 3 * TN, NS = OC + 3;
+__global__ __launch_bounds__(256) void k_gather_faces(Box b, const T *__restrict__ face, T *__restrict__ y) {
+  constexpr int F2 = TN * TN, UP = 2 * BE;
+  constexpr int OFX = 0, OFY = 3 * F2, OFZ = 6 * F2, OXY = 9 * F2, OXZ = OXY + 3 * TN, OYZ = OXZ + 3 * TN,
+                OC = OYZ + 3 * TN, NS = OC + 3;
   __shared__ T sh[NS];
   const BrickId br = brick_xcd(b, blockIdx.x);
-  const Tile t = tile_of(b, br);
-  const int64_t lin = br.lin, sx = TILEVALS, sy = (int64_t)b.nbx * TILEVALS, sz = (int64_t)b.nbx * b.nby * TILEVALS;
   const bool hx = br.bx > 0, hy = br.by > 0, hz = br.bz > 0;
-  const T *own = buf + lin * TILEVALS;
-  // every load is issued before the first LDS store (one exposed latency per block); the
-  // neighbour addresses fall back to the own slot where a neighbour does not exist
-  constexpr int NV = (3 * TV3 + 255) / 256;
+  if (!(hx || hy || hz)) return;
+  const Tile t = tile_of(b, br);
   const int tid = threadIdx.x;
-  T v[NV];
-#pragma unroll
-  for (int m = 0; m < NV; m++) v[m] = own[min(tid + 256 * m, 3 * TV3 - 1)];
-  const int g = min(tid, 3 * F2 - 1);
-  const int gc = g / F2, gr = g - gc * F2, gq = gr / TN, gp = gr - gq * TN;  // gr = gq*TN + gp
-  const T fx = own[(hx ? -sx : 0) + gc * TV3 + gr * TN + UP];              // (UP, iy=gp, iz=gq)
-  const T fy = own[(hy ? -sy : 0) + gc * TV3 + (gq * TN + UP) * TN + gp];  // (ix=gp, UP, iz=gq)
-  const T fz = own[(hz ? -sz : 0) + gc * TV3 + (UP * TN + gq) * TN + gp];  // (ix=gp, iy=gq, UP)
+  const int64_t fx = FACEVALS, fy = (int64_t)b.nbx * FACEVALS, fz = (int64_t)b.nbx * b.nby * FACEVALS;
+  const T *fb = face + (int64_t)br.lin * FACEVALS;  // own slot: the fallback address of absent neighbours
+  // slot value (plane, row a, column b, component c) sits at 3 * (plane * F2 + a * TN + b) + c
+  const int g = min(tid, 3 * F2 - 1), gc = g / F2, gr = g - gc * F2;
+  const T vx = fb[(hx ? -fx : 0) + 3 * gr + gc];           // X plane of bx-1: (UP, y, z), gr = z*TN + y
+  const T vy = fb[(hy ? -fy : 0) + 3 * (F2 + gr) + gc];    // Y plane of by-1: (x, UP, z), gr = z*TN + x
+  const T vz = fb[(hz ? -fz : 0) + 3 * (2 * F2 + gr) + gc];  // Z plane of bz-1: (x, y, UP), gr = y*TN + x
   const int e = min(tid, 3 * TN - 1), ec = e / TN, ep = e - ec * TN;
-  const T exy = own[(hx && hy ? -sx - sy : 0) + ec * TV3 + (ep * TN + UP) * TN + UP];  // (UP, UP, iz=ep)
-  const T exz = own[(hx && hz ? -sx - sz : 0) + ec * TV3 + (UP * TN + ep) * TN + UP];  // (UP, iy=ep, UP)
-  const T eyz = own[(hy && hz ? -sy - sz : 0) + ec * TV3 + (UP * TN + UP) * TN + ep];  // (ix=ep, UP, UP)
-  const int cc = min(tid, 2);
-  const T cxyz = own[(hx && hy && hz ? -sx - sy - sz : 0) + cc * TV3 + (UP * TN + UP) * TN + UP];
-#pragma unroll
-  for (int m = 0; m < NV; m++)
-    if (tid + 256 * m < 3 * TV3) sh[tid + 256 * m] = v[m];
+  const T exy = fb[(hx && hy ? -fx - fy : 0) + 3 * (ep * TN + UP) + ec];          // X plane: (UP, UP, z = ep)
+  const T exz = fb[(hx && hz ? -fx - fz : 0) + 3 * (UP * TN + ep) + ec];          // X plane: (UP, y = ep, UP)
+  const T eyz = fb[(hy && hz ? -fy - fz : 0) + 3 * (F2 + UP * TN + ep) + ec];     // Y plane: (x = ep, UP, UP)
+  const T cxyz = fb[(hx && hy && hz ? -fx - fy - fz : 0) + 3 * (UP * TN + UP) + min(tid, 2)];  // X plane corner
   if (tid < 3 * F2) {
-    sh[OFX + tid] = fx;
-    sh[OFY + tid] = fy;
-    sh[OFZ + tid] = fz;
+    sh[OFX + tid] = vx;
+    sh[OFY + tid] = vy;
+    sh[OFZ + tid] = vz;
   }
   if (tid < 3 * TN) {
     sh[OXY + tid] = exy;
@@ -948,48 +940,63 @@ __global__ __launch_bounds__(256) void k_gather(Box b, const T *__restrict__ buf
   }
   if (tid < 3) sh[OC + tid] = cxyz;
   __syncthreads();
+  // the brick's nodes on its lower faces: plane x = 0, then y = 0 (x >= x0), then z = 0
+  // (x >= x0, y >= y0), each node once
   const int ox = br.bx == b.nbx - 1 ? t.tnx : UP;
   const int oy = br.by == b.nby - 1 ? t.tny : UP;
   const int oz = br.bz == b.nbz - 1 ? t.tnz : UP;
-  // thread = (row group rg, interlaced column k = 3 ix + c); rows (iy, iz) step by TN
-  const int rg = threadIdx.x / RLEN, k = threadIdx.x - RLEN * rg;
-  const int ix = k / 3, c = k - 3 * ix;
-  if (rg >= TN || ix >= ox) return;
-  const bool nx = hx && ix == 0;
-  T *yb = y + 3 * t.node0 + k;
-  const int64_t ys = 3 * (int64_t)b.NX, zs = 3 * (int64_t)b.NX * b.NY;
-  int iy = rg, iz = 0;
-  while (iy >= oy) { iy -= oy; iz++; }
-  while (iz < oz) {
-    const bool ny = hy && iy == 0, nz = hz && iz == 0;
-    // lower neighbours in increasing brick index, the own slot last
-    T acc = T(0);
+  const int x0 = hx ? 1 : 0, y0 = hy ? 1 : 0;
+  const int nX = hx ? oy * oz : 0, nY = hy ? (ox - x0) * oz : 0, nZ = hz ? (ox - x0) * (oy - y0) : 0;
+  const int nval = 3 * (nX + nY + nZ);
+  int64_t gi[3];
+  int ix[3], iy[3], iz[3], cc[3];
+  T own[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    const int v = min(tid + 256 * m, nval - 1);
+    const int n = v / 3;
+    cc[m] = v - 3 * n;
+    if (n < nX) {
+      ix[m] = 0; iy[m] = n % oy; iz[m] = n / oy;
+    } else if (n < nX + nY) {
+      const int q = n - nX;
+      ix[m] = x0 + q % (ox - x0); iy[m] = 0; iz[m] = q / (ox - x0);
+    } else {
+      const int q = n - nX - nY;
+      ix[m] = x0 + q % (ox - x0); iy[m] = y0 + q / (ox - x0); iz[m] = 0;
+    }
+    gi[m] = 3 * (t.node0 + ix[m] + (int64_t)b.NX * (iy[m] + (int64_t)b.NY * iz[m])) + cc[m];
+    own[m] = y[gi[m]];
+  }
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    if (tid + 256 * m >= nval) break;
+    const int c = cc[m], X = ix[m], Y = iy[m], Z = iz[m];
+    const bool nx = hx && X == 0, ny = hy && Y == 0, nz = hz && Z == 0;
+    T acc = own[m];
     if (nz && ny && nx) acc += sh[OC + c];
-    if (nz && ny) acc += sh[OYZ + c * TN + ix];
-    if (nz && nx) acc += sh[OXZ + c * TN + iy];
-    if (nz) acc += sh[OFZ + c * F2 + iy * TN + ix];
-    if (ny && nx) acc += sh[OXY + c * TN + iz];
-    if (ny) acc += sh[OFY + c * F2 + iz * TN + ix];
-    if (nx) acc += sh[OFX + c * F2 + iz * TN + iy];
-    acc += sh[c * TV3 + (iz * TN + iy) * TN + ix];
-    T *py = yb + iy * ys + iz * zs;
-    *py = add_mode ? *py + acc : acc;
-    iy += TN;
-    while (iy >= oy) { iy -= oy; iz++; }
+    if (nz && ny) acc += sh[OYZ + c * TN + X];
+    if (nz && nx) acc += sh[OXZ + c * TN + Y];
+    if (nz) acc += sh[OFZ + c * F2 + Y * TN + X];
+    if (ny && nx) acc += sh[OXY + c * TN + Z];
+    if (ny) acc += sh[OFY + c * F2 + Z * TN + X];
+    if (nx) acc += sh[OFX + c * F2 + Z * TN + Y];
+    y[gi[m]] = acc;
   }
 }
 
 template <typename T>
-hipError_t launch_gather(const Box &b, const T *buf, T *y, bool add, hipStream_t s) {
-  hipLaunchKernelGGL(k_gather<T>, dim3(nbricks(b)), dim3(256), 0, s, b, buf, y, add ? 1 : 0);
+hipError_t launch_gather(const Box &b, const T *face, T *y, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_faces<T>, dim3(nbricks(b)), dim3(256), 0, s, b, face, y);
   return hipGetLastError();
 }
 
 hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
-                           const double *eta_bm, double *y, bool add, hipStream_t s, double *tilebuf) {
-  if (tilebuf) {
-    hipLaunchKernelGGL(k_a11_fp64, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm, y, 0, tilebuf);
-    return launch_gather(b, (const double *)tilebuf, y, add, s);
+                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf) {
+  if (facebuf) {
+    hipLaunchKernelGGL(k_a11_fp64, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm, y, add ? 1 : 0,
+                       facebuf);
+    return launch_gather(b, (const double *)facebuf, y, s);
   }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
@@ -1001,10 +1008,10 @@ hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, 
 }
 
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
-                           hipStream_t s, float *tilebuf) {
-  if (tilebuf) {
-    hipLaunchKernelGGL(k_a11_fp32, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, fac, y, 0, tilebuf);
-    return launch_gather(b, (const float *)tilebuf, y, add, s);
+                           hipStream_t s, float *facebuf) {
+  if (facebuf) {
+    hipLaunchKernelGGL(k_a11_fp32, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, fac, y, add ? 1 : 0, facebuf);
+    return launch_gather(b, (const float *)facebuf, y, s);
   }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
@@ -1021,10 +1028,11 @@ hipError_t launch_build_fac32(const Box &b, const double *coords, const double *
 }
 
 hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,
-                            double *y, bool add, hipStream_t s, double *tilebuf) {
-  if (tilebuf) {
-    hipLaunchKernelGGL(k_a11_fp64g, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, geo, eta_bm, y, 0, tilebuf);
-    return launch_gather(b, (const double *)tilebuf, y, add, s);
+                            double *y, bool add, hipStream_t s, double *facebuf) {
+  if (facebuf) {
+    hipLaunchKernelGGL(k_a11_fp64g, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, geo, eta_bm, y, add ? 1 : 0,
+                       facebuf);
+    return launch_gather(b, (const double *)facebuf, y, s);
   }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
@@ -1041,11 +1049,17 @@ hipError_t launch_build_geo64(const Box &b, const double *coords, double *geo, h
 }
 
 hipError_t launch_stokes_fp32(const Box &b, const float *u, const float *p, const uint8_t *mask, const float *fac,
-                              const float *facb, float *yu, float *yp, bool add, hipStream_t s) {
+                              const float *facb, float *yu, float *yp, bool add, hipStream_t s, float *facebuf) {
+  if (facebuf) {
+    hipLaunchKernelGGL(k_stokes_fp32, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, p, mask, fac, facb, yu, yp,
+                       add ? 1 : 0, facebuf);
+    return launch_gather(b, (const float *)facebuf, yu, s);
+  }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
-    hipLaunchKernelGGL(k_stokes_fp32, dim3(nb), dim3(BEL), 0, s, b, c, u, p, mask, fac, facb, yu, yp, add ? 1 : 0);
+    hipLaunchKernelGGL(k_stokes_fp32, dim3(nb), dim3(BEL), 0, s, b, c, u, p, mask, fac, facb, yu, yp, add ? 1 : 0,
+                       (float *)nullptr);
   }
   return hipGetLastError();
 }

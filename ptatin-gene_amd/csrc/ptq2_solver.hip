@@ -523,21 +523,18 @@ int level_matmult(ptq2_mg *mg, int l, const T *x, T *y);
 template <>
 int level_matmult<double>(ptq2_mg *mg, int l, const double *x, double *y) {
   ptq2_ctx *c = mg->ctx[l];
-  // the level's stored geometry and output path were prepared at set-up (no allocation
-  // here: this runs under graph capture)
-  if (c->geo64_valid)
-    MGCHK(ptq2::launch_a11_fp64g(c->box, x, c->mask, c->geo64, c->eta_bm, y, false, mg->stream,
-                                 (double *)c->tilebuf));
-  else
-    MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream,
-                                (double *)c->tilebuf));
+  // geometry recomputed from the coordinates (the faster FP64 kernel); the level's face
+  // buffer was allocated by its set-up applies (no allocation here: this runs under graph
+  // capture)
+  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream,
+                              (double *)c->facebuf));
   MGCHK(ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
   return PTQ2_OK;
 }
 template <>
 int level_matmult<float>(ptq2_mg *mg, int l, const float *x, float *y) {
   ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->tilebuf));
+  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf));
   MGCHK(ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
   return PTQ2_OK;
 }

@@ -25,6 +25,7 @@ FP32 = 0x2
 MATMULT = 0x4
 ADD = 0x8
 ONTHEFLY = 0x10
+STOREDGEO = 0x20
 
 # every entry point of include/ptq2.h with its ctypes signature
 _c_int, _c_uint, _vp, _i64 = ctypes.c_int, ctypes.c_uint, ctypes.c_void_p, ctypes.c_int64

@@ -73,8 +73,8 @@ def test_a11_fp64_sinker_matmult(sinker):
 
 @pytest.mark.parametrize("dims", [(20, 20, 20), (9, 4, 5), (1, 1, 3)])
 def test_a11_fp64_geometry_variants(dims):
-    """The resident-geometry kernel (default) and the on-the-fly kernel (PTQ2_ONTHEFLY)
-    against the oracle, MatMult and raw ADD, including partial bricks."""
+    """The on-the-fly kernel (default, PTQ2_ONTHEFLY) and the resident-geometry kernel
+    (PTQ2_STOREDGEO) against the oracle, MatMult and raw ADD, including partial bricks."""
     mx, my, mz = dims
     say("case: A11 FP64 stored vs on-the-fly geometry %s" % (dims,))
     c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 3)
@@ -90,7 +90,7 @@ def test_a11_fp64_geometry_variants(dims):
     y0 = M.uniform(16, n, -1.0, 1.0)
     ref = O.matmult_a11(mx, my, mz, c, eta, mask, u)
     raw = y0 + O.a11_tensor(mx, my, mz, c, eta, u, threads=8)
-    for fl in (0, _lib.ONTHEFLY):
+    for fl in (0, _lib.ONTHEFLY, _lib.STOREDGEO):
         assert rel(ctx.a11(u, flags=_lib.MATMULT | fl), ref) <= 1e-12
         assert rel(ctx.a11(u, y0.copy(), flags=_lib.ADD | fl), raw) <= 1e-12
 
@@ -274,6 +274,7 @@ def test_error_paths():
     # bad flags
     assert L.ptq2_a11_apply(ctx.handle, u.ctypes.data, y.ctypes.data, 0x100) == _lib.ERR_INVALID_ARG
     assert L.ptq2_a11_apply(ctx.handle, u.ctypes.data, y.ctypes.data, _lib.ADD | _lib.MATMULT) == _lib.ERR_INVALID_ARG
+    assert L.ptq2_a11_apply(ctx.handle, u.ctypes.data, y.ctypes.data, _lib.ONTHEFLY | _lib.STOREDGEO) == _lib.ERR_INVALID_ARG
     # inverted element: swap two node planes of one element
     c = P.coords.reshape(2 * m + 1, 2 * m + 1, 2 * m + 1, 3).copy()
     c[2, 2, 2, 0], c[2, 2, 4, 0] = c[2, 2, 4, 0], c[2, 2, 2, 0]
