@@ -73,6 +73,58 @@ def time_applies(fn, steps, warmup, stream):
     return t0.elapsed_time(t1) / steps  # ms per step
 
 
+def copy_bandwidth(stream, nbytes=1 << 30, reps=10):
+    """Device-to-device copy of nbytes (read + write counted), the measured HBM ceiling next to
+    the 8 TB/s peak."""
+    import torch
+
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    ms = time_applies(lambda: dst.copy_(src), reps, 2, stream)
+    del src, dst
+    return round(2 * nbytes / (ms * 1e-3) / 1e9, 1)
+
+
+def block_comparison(m, steps, warmup, stream):
+    """The full Stokes block [A11 A12; A21 0] on the sinker with viscosity contrast 1e6 (no
+    noise), FP64 and FP32 (one fused pass on FP32-stored factors), random velocity and
+    pressure: time per apply and the FP32 result's relative error against FP64."""
+    import torch
+
+    import ptq2
+    from ptq2 import _lib
+    from ptq2 import mesh as M
+
+    P = M.config_sinker(m, n_incl=12, contrast=1e6, noise=False, bc="none")
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(m, m, m, P.coords, P.mask)
+    ctx.set_viscosity(P.eta)
+    del P
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(7)
+    u = torch.rand(ctx.nvel, dtype=torch.float64, device="cuda:0", generator=g) * 2 - 1
+    p = torch.rand(ctx.npres, dtype=torch.float64, device="cuda:0", generator=g) * 2 - 1
+    yu, yp = torch.empty_like(u), torch.empty_like(p)
+    u32, p32 = u.float(), p.float()
+    yu32, yp32 = torch.empty_like(u32), torch.empty_like(p32)
+    ctx.stokes(u32, p32, yu32, yp32, _lib.FP32)  # builds the FP32 factors
+    ms64 = time_applies(lambda: ctx.stokes(u, p, yu, yp, 0), steps, warmup, stream)
+    ms32 = time_applies(lambda: ctx.stokes(u32, p32, yu32, yp32, _lib.FP32), steps, warmup, stream)
+    ctx.stokes(u, p, yu, yp, 0)
+    ctx.stokes(u32, p32, yu32, yp32, _lib.FP32)
+    torch.cuda.synchronize()
+    err_u = float(torch.linalg.norm(yu32.double() - yu) / torch.linalg.norm(yu))
+    err_p = float(torch.linalg.norm(yp32.double() - yp) / torch.linalg.norm(yp))
+    ndof = ctx.nvel + ctx.npres
+    out = {"workload": "full Stokes block [A11 A12; A21 0], %d^3 Q2-P1disc elements (%d velocity + %d pressure "
+                       "dofs), 12 sinker inclusions eta 1e6 in eta 1, random u and p" % (m, ctx.nvel, ctx.npres),
+           "fp64_ms": round(ms64, 4), "fp64_gdofs": round(ndof / (ms64 * 1e-3) / 1e9, 3),
+           "fp32_ms": round(ms32, 4), "fp32_gdofs": round(ndof / (ms32 * 1e-3) / 1e9, 3),
+           "fp32_rel_err_u": err_u, "fp32_rel_err_p": err_p}
+    ctx.destroy()
+    return out
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle's restatement of the reference's default sum-factorized CPU algorithm
     (OpenMP over race-free element colours) on a bounded 40^3 sample."""
@@ -99,7 +151,7 @@ def cpu_baseline(seconds=12.0):
             "sample": "%d FP64 A11 applies on a 40^3-element Q2 mesh (%d velocity dofs), %.1f s" % (n, P.nvel, el)}
 
 
-def solve_comparison(m, nlevels, rtol=1e-8, restart=120, max_it=1000, emin_frac=0.05):
+def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=0.05):
     """The viscous solve of the buoyancy problem (free-slip walls, 20 dense spheres with
     viscosity contrast 1e4) with GMG-preconditioned FGMRES, once with FP64 and once with FP32
     Chebyshev smoothers.  Wall time of each solve from x = 0 to rtol on the host clock
@@ -205,6 +257,8 @@ def main():
                     help="also time the FP32- vs FP64-smoothed GMG solve on an M^3 mesh (0 = skip)")
     ap.add_argument("--solve-levels", type=int, default=5)
     ap.add_argument("--batch", action="store_true", help="also time 8 uneven solves, sequential vs concurrent")
+    ap.add_argument("--block", type=int, default=1, metavar="0|1",
+                    help="time the full Stokes block (FP64 and FP32) on the contrast-1e6 sinker (default 1)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -260,11 +314,17 @@ def main():
         torch.cuda.synchronize()
         extra["fp32_rel_err_vs_fp64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
 
+    extra["hbm_copy_gbs"] = copy_bandwidth(stream)
+    # release the apply set-up before the larger comparisons
+    del A, u, y
+    u32 = y32 = None
+    torch.cuda.empty_cache()
+    if args.block:
+        try:
+            extra["block"] = block_comparison(m, args.steps, args.warmup, stream)
+        except Exception as exc:
+            extra["block"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     if args.solve:
-        del A, u, y
-        if not args.no_fp32:
-            del u32, y32
-        torch.cuda.empty_cache()
         try:
             extra["solve"] = solve_comparison(args.solve, args.solve_levels)
         except Exception as exc:  # the apply line above stays valid; report the failure

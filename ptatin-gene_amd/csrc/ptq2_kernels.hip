@@ -432,13 +432,21 @@ __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const flo
                                                      const float *__restrict__ fac, const float *__restrict__ facb,
                                                      float *__restrict__ yu, float *__restrict__ yp, int add_mode,
                                                      float *__restrict__ facebuf) {
-  __shared__ double Ts[3 * TCS];  // FP32 velocity tile during the rows, FP64 output tile after
+  // LDS: the FP32 velocity tile and one quadrature row of A11 factors ([qa*9+m][lane]) and
+  // block factors ([qa*4+n][lane]) during the rows, then the FP64 output tile
+  constexpr int FROW = 27 * BEL, BROW = 12 * BEL;
+  constexpr int NTS = (sizeof(float) * (3 * TCS + FROW + BROW) + sizeof(double) - 1) / sizeof(double);
+  static_assert(NTS >= 3 * TCS, "the output tile fits");
+  __shared__ double Ts[NTS];
   float *Us = reinterpret_cast<float *>(Ts);
+  float *Fs = Us + 3 * TCS, *Bs = Fs + FROW;
   const int lane = threadIdx.x;
   const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
-  const float *fbase = fac + (int64_t)br.lin * 243 * BEL + lane;
-  const float *bbase = facb + (int64_t)br.lin * 108 * BEL + lane;
+  const float *fsrc = fac + (int64_t)br.lin * 243 * BEL;
+  const float *bsrc = facb + (int64_t)br.lin * 108 * BEL;
+  lds_copy_async<FROW * 4>(fsrc, Fs, lane);
+  lds_copy_async<BROW * 4>(bsrc, Bs, lane);
   load_tile(b, t, br, u, mask, Us, lane);
   __syncthreads();
   const int base = lane_base(lane);
@@ -454,18 +462,20 @@ __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const flo
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
   float Yp[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-  if (active) {
-    const float pe[4] = {p[4 * e], p[4 * e + 1], p[4 * e + 2], p[4 * e + 3]};
+  // every lane runs the rows (the copies need the whole wave); inactive lanes of a partial
+  // brick read no pressure and their outputs are discarded
+  {
+    float pe[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (active) {
+#pragma unroll
+      for (int n = 0; n < 4; n++) pe[n] = p[4 * e + n];
+    }
     float ZB[3][3][3], ZD[3][3][3];
 #pragma nounroll
     for (int row = 0; row < 9; row++) {
       const int qb = row % 3, qc = row / 3;
       const Coef1 cb = coef1(qb), cc = coef1(qc);
-      float F[27], B4[12];
-#pragma unroll
-      for (int n = 0; n < 27; n++) F[n] = fbase[(int64_t)(27 * row + n) * BEL];
-#pragma unroll
-      for (int n = 0; n < 12; n++) B4[n] = bbase[(int64_t)(12 * row + n) * BEL];
+      __builtin_amdgcn_s_waitcnt(WAIT_VM0);
       if (qb == 0) slab_z32(Us, base, cc, ZB, ZD);
       float ur0[3][3], ur1[3][3], ur2[3][3];
 #pragma unroll
@@ -478,16 +488,33 @@ __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const flo
         }
       float P0[3][3], P1[3][3], P2[3][3];
       zero33(P0); zero33(P1); zero33(P2);
-      stokes_qp32<0>(ur0, ur1, ur2, F, B4, pe, P0, P1, P2, Yp);
-      stokes_qp32<1>(ur0, ur1, ur2, F + 9, B4 + 4, pe, P0, P1, P2, Yp);
-      stokes_qp32<2>(ur0, ur1, ur2, F + 18, B4 + 8, pe, P0, P1, P2, Yp);
+      // each point's factors come from LDS just before use; the next row's copy starts once
+      // the three points have read theirs
+#pragma unroll
+      for (int qa = 0; qa < 3; qa++) {
+        float F9[9], B4[4];
+#pragma unroll
+        for (int n = 0; n < 9; n++) F9[n] = Fs[(9 * qa + n) * BEL + lane];
+#pragma unroll
+        for (int n = 0; n < 4; n++) B4[n] = Bs[(4 * qa + n) * BEL + lane];
+        if (qa == 0) stokes_qp32<0>(ur0, ur1, ur2, F9, B4, pe, P0, P1, P2, Yp);
+        if (qa == 1) stokes_qp32<1>(ur0, ur1, ur2, F9, B4, pe, P0, P1, P2, Yp);
+        if (qa == 2) stokes_qp32<2>(ur0, ur1, ur2, F9, B4, pe, P0, P1, P2, Yp);
+      }
+      if (row < 8) {
+        __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+        lds_copy_async<FROW * 4>(fsrc + (row + 1) * FROW, Fs, lane);
+        lds_copy_async<BROW * 4>(bsrc + (row + 1) * BROW, Bs, lane);
+      }
       row_fence();
       acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
       row_fence();
     }
-    float *o = yp + 4 * e;
+    if (active) {
+      float *o = yp + 4 * e;
 #pragma unroll
-    for (int i = 0; i < 4; i++) o[i] = add_mode ? o[i] + Yp[i] : Yp[i];
+      for (int i = 0; i < 4; i++) o[i] = add_mode ? o[i] + Yp[i] : Yp[i];
+    }
   }
   __syncthreads();
   zero_tile(Ts, lane);

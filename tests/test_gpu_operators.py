@@ -322,6 +322,14 @@ def test_full_size_properties_128():
     assert abs(s1 - s2) <= 1e-11 * float(torch.linalg.norm(Ax) * torch.linalg.norm(z))
     Ax32 = A.ctx.a11(x.float(), flags=_lib.FP32)
     assert float(torch.linalg.norm(Ax32.double() - Ax) / torch.linalg.norm(Ax)) <= 1e-5
+    # full block at full size (contrast 1e6): the fused FP32 pass against the FP64 passes
+    say("case: 128^3 full block FP32 vs FP64")
+    pv = torch.rand(4 * m ** 3, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
+    yu, yp = A.ctx.stokes(x, pv)
+    yu32, yp32 = A.ctx.stokes(x.float(), pv.float(), flags=_lib.FP32)
+    assert float(torch.linalg.norm(yu32.double() - yu) / torch.linalg.norm(yu)) <= 1e-5
+    assert float(torch.linalg.norm(yp32.double() - yp) / torch.linalg.norm(yp)) <= 1e-5
+    del yu, yp, yu32, yp32, pv
     cx = torch.from_numpy(c).to(DEV).view(-1, 3)
     rot = torch.stack([-cx[:, 1], cx[:, 0], torch.zeros_like(cx[:, 0])], dim=1).reshape(-1).contiguous()
     assert float(torch.linalg.norm(A.ctx.a11(rot))) <= 1e-11 * float(torch.linalg.norm(Ax))
