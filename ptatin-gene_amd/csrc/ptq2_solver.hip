@@ -41,6 +41,37 @@ __global__ void k_fill(int64_t n, T *x, T v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) x[i] = v;
 }
 
+// Input and output of the captured V-cycle: the graph's first kernel reads its input vector and
+// scale, its last kernel its output vector, through this record, so one graph serves any
+// caller vectors without copies (the record is set by k_set_io before each launch)
+struct VcIO {
+  const double *in;
+  double *out;
+  double sin, sout;
+};
+__global__ void k_set_io(VcIO *io, const double *in, double *out, double sin, double sout) {
+  if (threadIdx.x == 0) {
+    io->in = in;
+    io->out = out;
+    io->sin = sin;
+    io->sout = sout;
+  }
+}
+template <typename T>
+__global__ void k_vin(const VcIO *__restrict__ io, int64_t n, T *__restrict__ b) {
+  const double *in = io->in;
+  const double s = io->sin;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = static_cast<T>(s * in[i]);
+}
+template <typename T>
+__global__ void k_vout(const VcIO *__restrict__ io, int64_t n, const T *__restrict__ x) {
+  double *out = io->out;
+  const double s = io->sout;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = s * static_cast<double>(x[i]);
+}
+
 template <typename S, typename D>
 __global__ void k_convert(int64_t n, const S *__restrict__ a, D *__restrict__ b) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -477,7 +508,7 @@ struct ptq2_mg {
   std::vector<double> emax, emin;
   double *ainv = nullptr;               // dense inverse of the coarsest operator
   int ainv_ld = 0;                      // its leading dimension (padded to the sweep block)
-  double *r_in = nullptr, *z_out = nullptr;  // FP64 graph input/output (level 0)
+  VcIO *io = nullptr;                   // input/output record of the V-cycle (level 0, FP64 vectors)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   // Krylov workspace
@@ -602,19 +633,21 @@ int vcycle_level(ptq2_mg *mg, int l) {
   return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle launch");
 }
 
-// the work of one V-cycle: r_in (FP64) -> z_out (FP64)
+// the work of one V-cycle: io->out = io->sout * M^-1 (io->sin * io->in), FP64 vectors
 int vcycle_body(ptq2_mg *mg) {
   const int64_t n = mg->n[0];
   if (mg->f32) {
-    hipLaunchKernelGGL((k_convert<double, float>), dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)mg->r_in,
+    hipLaunchKernelGGL(k_vin<float>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
                        (float *)mg->vb[0]);
     MGRC(vcycle_level<float>(mg, 0));
-    hipLaunchKernelGGL((k_convert<float, double>), dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const float *)mg->vx[0],
-                       mg->z_out);
+    hipLaunchKernelGGL(k_vout<float>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
+                       (const float *)mg->vx[0]);
   } else {
-    MGCHK(hipMemcpyAsync(mg->vb[0], mg->r_in, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+    hipLaunchKernelGGL(k_vin<double>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
+                       (double *)mg->vb[0]);
     MGRC(vcycle_level<double>(mg, 0));
-    MGCHK(hipMemcpyAsync(mg->z_out, mg->vx[0], sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+    hipLaunchKernelGGL(k_vout<double>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
+                       (const double *)mg->vx[0]);
   }
   return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle");
 }
@@ -822,17 +855,13 @@ __global__ void k_negate(int64_t n, double *__restrict__ x) {
 }
 
 
-// z = M^-1 (scale r): one V-cycle on the scaled input
+// z = M^-1 (scale r): one V-cycle reading r and writing z directly (the input is read in full
+// by the graph's first kernel before its last kernel writes z, so r and z may alias)
 int vcycle_scaled(ptq2_mg *mg, const double *r, double scale, double *z) {
-  const int64_t n = mg->n[0];
-  if (scale == 1.0)
-    MGCHK(hipMemcpyAsync(mg->r_in, r, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
-  else
-    hipLaunchKernelGGL(k_scale_copy, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, r, scale, mg->r_in);
+  hipLaunchKernelGGL(k_set_io, dim3(1), dim3(64), 0, mg->stream, mg->io, r, z, scale, 1.0);
   if (mg->exec) MGCHK(hipGraphLaunch(mg->exec, mg->stream));
   else MGRC(vcycle_body(mg));
-  MGCHK(hipMemcpyAsync(z, mg->z_out, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
-  return PTQ2_OK;
+  return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle launch");
 }
 
 }  // namespace
@@ -868,7 +897,7 @@ int ptq2_mg_destroy(ptq2_mg **pmg) {
   if (!mg->ctx.empty() && mg->ctx[0]) ptq2_set_stream(mg->ctx[0], mg->fine_stream0);
   for (double *p : mg->Vk) (void)hipFree(p);
   for (double *p : mg->Zk) (void)hipFree(p);
-  for (void *p : {(void *)mg->ainv, (void *)mg->r_in, (void *)mg->z_out, (void *)mg->w, (void *)mg->part, (void *)mg->psum,
+  for (void *p : {(void *)mg->ainv, (void *)mg->io, (void *)mg->w, (void *)mg->part, (void *)mg->psum,
                   (void *)mg->hdev, (void *)mg->Vptr, (void *)mg->Zptr})
     if (p) (void)hipFree(p);
   delete mg;
@@ -1103,8 +1132,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     if (herr) return mgfail(mg, PTQ2_ERR_MESH, "coarse operator is not positive definite");
   }
   // ---- graph of one V-cycle
-  MGRC(mg_alloc(mg, (void **)&mg->r_in, sizeof(double) * mg->n[0]));
-  MGRC(mg_alloc(mg, (void **)&mg->z_out, sizeof(double) * mg->n[0]));
+  MGRC(mg_alloc(mg, (void **)&mg->io, sizeof(VcIO)));
   if (o.use_graph) {
     MGCHK(hipStreamBeginCapture(mg->stream, hipStreamCaptureModeThreadLocal));
     int rc = vcycle_body(mg);
