@@ -199,7 +199,7 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
     return out
 
 
-def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1000, restart=120):
+def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1500, restart=256):
     """Independent buoyancy solves of uneven sizes (viscosity contrasts 1e2..1e6, seeded
     sphere placement, FP32 smoothing), run one after another and then concurrently (one
     stream and host thread per problem).  Host wall time of each variant."""
