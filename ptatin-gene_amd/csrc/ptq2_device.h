@@ -7,9 +7,12 @@
 // registers (no cross-lane traffic), accumulates its 81 outputs into an LDS
 // output tile with ds_add_f64 (each instruction's 64 addresses are distinct,
 // so the sum order is the fixed program order), and the tile is written back
-// with row-contiguous stores.  Bricks are launched in 8 colours
-// (brick parity in x, y, z): bricks of one colour share no node, so the
-// write-back is a plain load-add-store and the result is bitwise reproducible.
+// with row-contiguous stores.  All bricks run in one launch: a brick stores the
+// nodes it owns and parks its shared upper-face values in a face slot that a
+// second pass adds into the owners (store_tile_own).  The older path, kept for
+// callers without a face buffer, launches bricks in 8 colours (brick parity in
+// x, y, z) so that bricks of one launch share no node (store_tile).  Either way
+// the sum order is fixed and the result is bitwise reproducible.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -488,22 +491,6 @@ __device__ __forceinline__ void acc_row(const T dv[3][3], T P0[3][3], T P1[3][3]
     }
 }
 
-// y-transpose of a finished row into the slab accumulators
-template <int qb, typename T>
-__device__ __forceinline__ void acc_slab(const T P0[3][3], const T P1[3][3], const T P2[3][3], T RB[3][3][3],
-                                         T RD[3][3][3]) {
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        if (nzB(qb, j)) RB[c][j][i] = fmac(gB(qb, j), P0[c][i], RB[c][j][i]);
-        if (nzD(qb, j)) RB[c][j][i] = fmac(gD(qb, j), P1[c][i], RB[c][j][i]);
-        if (nzB(qb, j)) RD[c][j][i] = fmac(gB(qb, j), P2[c][i], RD[c][j][i]);
-      }
-}
-
 // x- and y-transpose of one quadrature point's dv straight into the slab
 // accumulators (no per-row accumulator kept live):
 //   RB[c][j][i] += B(qb,j) D(qa,i) dv[0][c] + D(qb,j) B(qa,i) dv[1][c]
@@ -585,28 +572,6 @@ __device__ __forceinline__ Coef1 coef1(int q) {
   return r;
 }
 
-// row data for runtime (qb,qc): z contraction first, then y.
-template <typename T>
-__device__ __forceinline__ void row_data_rt(const T *s, int base, const Coef1 &cb, const Coef1 &cc, T r0[3][3],
-                                            T r1[3][3], T r2[3][3]) {
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      T tb[3], td[3];
-#pragma unroll
-      for (int j = 0; j < 3; j++) {
-        T v0 = tv(s, base, c, i, j, 0), v1 = tv(s, base, c, i, j, 1), v2 = tv(s, base, c, i, j, 2);
-        tb[j] = fmac(cc.b[2], v2, fmac(cc.b[1], v1, static_cast<T>(cc.b[0]) * v0));
-        td[j] = fmac(cc.d[2], v2, fmac(cc.d[1], v1, static_cast<T>(cc.d[0]) * v0));
-      }
-      r0[c][i] = fmac(cb.b[2], tb[2], fmac(cb.b[1], tb[1], static_cast<T>(cb.b[0]) * tb[0]));
-      r1[c][i] = fmac(cb.d[2], tb[2], fmac(cb.d[1], tb[1], static_cast<T>(cb.d[0]) * tb[0]));
-      r2[c][i] = fmac(cb.b[2], td[2], fmac(cb.b[1], td[1], static_cast<T>(cb.b[0]) * td[0]));
-      __builtin_amdgcn_sched_barrier(0);
-    }
-}
-
 // One tile column: the 9 values (j,k) of component c at node column i.
 template <typename T>
 __device__ __forceinline__ void ld_col(const T *s, int base, int g, T v[9]) {
@@ -651,36 +616,6 @@ __device__ __forceinline__ void row_data_pipe(const T *s, const T *s_next, int b
   }
 }
 
-// y-transpose of a finished row (runtime qb) into the slab accumulators
-template <typename T>
-__device__ __forceinline__ void acc_slab_rt(const Coef1 &cb, const T P0[3][3], const T P1[3][3], const T P2[3][3],
-                                            T RB[3][3][3], T RD[3][3][3]) {
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int j = 0; j < 3; j++)
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        RB[c][j][i] = fmac(cb.b[j], P0[c][i], fmac(cb.d[j], P1[c][i], RB[c][j][i]));
-        RD[c][j][i] = fmac(cb.b[j], P2[c][i], RD[c][j][i]);
-      }
-}
-
-// z-transpose of a finished slab (runtime qc) into the element output
-template <typename T>
-__device__ __forceinline__ void acc_elem_rt(const Coef1 &cc, const T RB[3][3][3], const T RD[3][3][3],
-                                            T Y[3][3][3][3]) {
-#pragma unroll
-  for (int c = 0; c < 3; c++)
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-          Y[c][k][j][i] = fmac(cc.b[k], RB[c][j][i], fmac(cc.d[k], RD[c][j][i], Y[c][k][j][i]));
-}
-
 // a finished row (runtime qb,qc) straight into the element output: per (c,j,i) the
 // y-transposed pair is formed once and spread over the three z nodes.
 template <typename T>
@@ -717,11 +652,6 @@ __device__ __forceinline__ void acc_elem(const T RB[3][3][3], const T RD[3][3][3
         }
 }
 
-template <typename T, int N>
-__device__ __forceinline__ void zero(T (&a)[N]) {
-#pragma unroll
-  for (int i = 0; i < N; i++) a[i] = T(0);
-}
 template <typename T>
 __device__ __forceinline__ void zero33(T a[3][3]) {
 #pragma unroll

@@ -72,11 +72,6 @@ __global__ void k_vout(const VcIO *__restrict__ io, int64_t n, const T *__restri
     out[i] = s * static_cast<double>(x[i]);
 }
 
-template <typename S, typename D>
-__global__ void k_convert(int64_t n, const S *__restrict__ a, D *__restrict__ b) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    b[i] = static_cast<D>(a[i]);
-}
 
 // p1 = scale * Dinv * r0 + x, with r0 = b (zero guess) or b - y
 template <typename T>

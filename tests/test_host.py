@@ -48,6 +48,21 @@ def test_strerror_and_null_arguments():
     assert L.ptq2_setup_mesh(None, 1, 1, 1, None, None, None) == _lib.ERR_INVALID_ARG
 
 
+def test_solver_default_options_round_trip():
+    """ptq2_mg_default_opts fills the ctypes mirror of ptq2_mg_opts field by field: the
+    documented defaults read back in place (a layout mismatch would scramble them)."""
+    from ptq2 import solver as Sv
+
+    o = Sv.default_opts()
+    assert (o.nlevels, o.smooth_its, o.fp32_smoother, o.esteig_its, o.use_graph) == (5, 4, 1, 20, 1)
+    assert (o.emin_frac, o.emax_frac) == (0.2, 1.1)
+    assert (o.coarse_max_dofs, o.galerkin_coarse, o.coarse_eta_mean) == (20000, 1, Sv.ETA_MEANS["arithmetic"])
+    assert sorted(Sv.ETA_MEANS.values()) == [0, 1, 2, 3]
+    txt = open(os.path.join(REPO, "include", "ptq2_solver.h")).read()
+    for name, v in Sv.ETA_MEANS.items():
+        assert re.search(r"PTQ2_ETA_MEAN_%s\s*=\s*%d" % (name.upper(), v), txt), name
+
+
 def test_create_without_gpu_returns_code():
     """On a host without a GPU (this container) creating a context reports an error code."""
     import torch

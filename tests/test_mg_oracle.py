@@ -37,6 +37,28 @@ def test_eta_coarsening_constant_and_mean():
     assert abs(ec[0, 0] - e[0, 0, 0].mean()) < 1e-14
 
 
+def test_eta_coarsening_rules_are_ordered_and_exact():
+    """Harmonic <= geometric <= arithmetic <= maximum at every coarse point (the power-mean
+    inequality), equality for constant children, and a hand-computed two-value case."""
+    m = 4
+    eta = 10.0 ** M.uniform(5, 27 * m ** 3, -2, 4)
+    h, g, a, x = (G.eta_coarsen(m, m, m, eta, mode) for mode in ("harmonic", "geometric", "arithmetic", "max"))
+    assert np.all(h <= g * (1 + 1e-12)) and np.all(g <= a * (1 + 1e-12)) and np.all(a <= x * (1 + 1e-12))
+    assert np.all(h > 0)
+    for mode in ("harmonic", "geometric", "arithmetic", "max"):
+        assert np.allclose(G.eta_coarsen(m, m, m, np.full(27 * m ** 3, 3.5), mode), 3.5, rtol=1e-14)
+    # children of coarse element 0 split along x: the x=0 children hold 1, the x=1 children 4;
+    # its centre point (qa = qb = qc = 1) sees all 8 children, 4 of each value
+    e = np.ones((m, m, m, 27))
+    e[:, :, 1::2] = 4.0
+    ec = {mode: G.eta_coarsen(m, m, m, e.reshape(-1), mode).reshape(-1, 27)[0, 13]
+          for mode in ("harmonic", "geometric", "arithmetic", "max")}
+    assert abs(ec["arithmetic"] - 2.5) < 1e-14
+    assert abs(ec["geometric"] - 2.0) < 1e-14
+    assert abs(ec["harmonic"] - 1.0 / (0.5 * (1.0 + 0.25))) < 1e-14
+    assert ec["max"] == 4.0
+
+
 def test_chebyshev_reduces_error_like_the_polynomial():
     """On a level with exact eigen-bounds the Chebyshev smoother's error reduction obeys the
     Chebyshev bound 1/T_k((emax+emin)/(emax-emin)) on the smoothed part of the spectrum."""
