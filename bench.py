@@ -151,7 +151,7 @@ def cpu_baseline(seconds=12.0):
             "sample": "%d FP64 A11 applies on a 40^3-element Q2 mesh (%d velocity dofs), %.1f s" % (n, P.nvel, el)}
 
 
-def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=0.05):
+def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=0.05, rigid=False):
     """The viscous solve of the buoyancy problem (free-slip walls, 20 dense spheres with
     viscosity contrast 1e4) with GMG-preconditioned FGMRES, once with FP64 and once with FP32
     Chebyshev smoothers.  Wall time of each solve from x = 0 to rtol on the host clock
@@ -174,11 +174,14 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
     print("solve set-up %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     out = {"workload": "A11 viscous solve, %d^3 Q2 elements, free-slip walls, 20 spheres eta 1e4 / rho 1.1, "
                        "buoyancy RHS, FGMRES(%d) + GMG V-cycle (%d levels, Chebyshev(4)-Jacobi on "
-                       "[%g, 1.1] lambda_max), rtol %g" % (m, restart, nlevels, emin_frac, rtol)}
+                       "[%g, 1.1] lambda_max)%s, rtol %g"
+                       % (m, restart, nlevels, emin_frac, ", rigid-body deflation of the inclusions" if rigid else "",
+                          rtol)}
     for fp32 in (False, True):
         key = "fp32" if fp32 else "fp64"
         t = time.perf_counter()
-        S = ViscousSolver(ctx, nlevels=nlevels, smooth_its=4, fp32=fp32, emin_frac=emin_frac)
+        S = ViscousSolver(ctx, nlevels=nlevels, smooth_its=4, fp32=fp32, emin_frac=emin_frac, rigid_deflation=rigid)
+        torch.cuda.synchronize()
         setup = time.perf_counter() - t
         S.vcycle(bd)  # warm the graph
         torch.cuda.synchronize()
@@ -199,10 +202,14 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
     return out
 
 
-def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1500, restart=256):
+def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1500, restart=256, rigid=True,
+                     emin_frac=0.05):
     """Independent buoyancy solves of uneven sizes (viscosity contrasts 1e2..1e6, seeded
     sphere placement, FP32 smoothing), run one after another and then concurrently (one
-    stream and host thread per problem).  Host wall time of each variant."""
+    stream and host thread per problem).  Host wall time of each variant.  Per problem also
+    the rounding floor of the FP64 residual: |A x - A' x| / |b|, A' the same operator
+    evaluated through the resident-geometry kernel (another rounding of the same sums); a
+    relative residual much below it is out of reach in FP64."""
     import torch
 
     import ptq2
@@ -218,7 +225,8 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
         ctx.setup_mesh(m, m, m, P.coords, P.mask)
         ctx.set_viscosity(P.eta)
         ctxs.append(ctx)
-        solvers.append(ViscousSolver(ctx, nlevels=auto_levels(m), smooth_its=4, fp32=True))
+        solvers.append(ViscousSolver(ctx, nlevels=auto_levels(m), smooth_its=4, fp32=True, rigid_deflation=rigid,
+                                     emin_frac=emin_frac))
         rhs.append(torch.from_numpy(b).to("cuda:0"))
     print("batch set-up %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     for S, b in zip(solvers, rhs):
@@ -232,8 +240,18 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
     bat = solve_batch(solvers, rhs, rtol=rtol, max_it=max_it, restart=restart)
     torch.cuda.synchronize()
     t_bat = time.perf_counter() - t
-    out = {"sizes": list(sizes), "contrasts": [float("%.3g" % c) for c in contrasts],
+    floor = []
+    for ctx, b, (x, _) in zip(ctxs, rhs, bat):
+        y1 = ctx.a11(x, flags=ptq2.MATMULT)
+        y2 = ctx.a11(x, flags=ptq2.MATMULT | 0x20)  # PTQ2_STOREDGEO
+        floor.append(float("%.3g" % (torch.linalg.norm(y1 - y2).item() / torch.linalg.norm(b).item())))
+        del y1, y2
+    out = {"workload": "8 buoyancy solves (20 spheres, rho 1.1, free-slip walls), FGMRES(%d) + GMG V-cycle "
+                       "(Chebyshev(4)-Jacobi FP32 on [%g, 1.1] lambda_max)%s, rtol %g, at most %d iterations"
+                       % (restart, emin_frac, ", rigid-body deflation of the inclusions" if rigid else "", rtol, max_it),
+           "sizes": list(sizes), "contrasts": [float("%.3g" % c) for c in contrasts],
            "iterations": [i["iterations"] for i in seq], "rel_residual": [i["rel"] for i in seq],
+           "fp64_residual_floor": floor,
            "batched_iterations_equal": [a["iterations"] == b[1]["iterations"] for a, b in zip(seq, bat)],
            "sequential_s": round(t_seq, 4), "concurrent_s": round(t_bat, 4),
            "speedup": round(t_seq / t_bat, 3)}
@@ -257,6 +275,11 @@ def main():
                     help="also time the FP32- vs FP64-smoothed GMG solve on an M^3 mesh (0 = skip)")
     ap.add_argument("--solve-levels", type=int, default=5)
     ap.add_argument("--batch", action="store_true", help="also time 8 uneven solves, sequential vs concurrent")
+    ap.add_argument("--rigid", choices=("off", "on", "both"), default="both",
+                    help="rigid-body deflation of the stiff inclusions in the 128^3 solve comparison (both: "
+                         "without it as 'solve', with it as 'solve_rigid')")
+    ap.add_argument("--batch-rigid", choices=("off", "on", "both"), default="on",
+                    help="rigid-body deflation in the batched solves (both: without it as 'batch_plain')")
     ap.add_argument("--block", type=int, default=1, metavar="0|1",
                     help="time the full Stokes block (FP64 and FP32) on the contrast-1e6 sinker (default 1)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -324,16 +347,19 @@ def main():
             extra["block"] = block_comparison(m, args.steps, args.warmup, stream)
         except Exception as exc:
             extra["block"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
-    if args.solve:
+    variants = {"off": [False], "on": [True], "both": [False, True]}
+    for rigid in variants[args.rigid] if args.solve else []:
+        key = "solve_rigid" if rigid and args.rigid == "both" else "solve"
         try:
-            extra["solve"] = solve_comparison(args.solve, args.solve_levels)
+            extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rigid)
         except Exception as exc:  # the apply line above stays valid; report the failure
-            extra["solve"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
-    if args.batch:
+            extra[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+    for rigid in variants[args.batch_rigid] if args.batch else []:
+        key = "batch_plain" if not rigid and args.batch_rigid == "both" else "batch"
         try:
-            extra["batch"] = batch_comparison()
+            extra[key] = batch_comparison(rigid=rigid)
         except Exception as exc:
-            extra["batch"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+            extra[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
 
     traffic = None
     prof = os.path.join(REPO, "profiles", "a11_fp64_128_traffic.json")

@@ -40,6 +40,17 @@ typedef struct {
   int coarse_eta_mean;  /* PTQ2_ETA_MEAN_*: how a coarse quadrature point combines the fine
                            quadrature viscosities of the child elements it touches (mean or
                            maximum) */
+  /* Two-level correction for stiff inclusions (no counterpart in the reference, whose
+     velocity solve is PCMG + FGMRES only; for high viscosity contrasts, where the rigid
+     motions of the inclusions stall the Krylov solve).  When the quadrature viscosity spans
+     at least rigid_min_contrast, an inclusion is a connected set of elements (sharing a node)
+     holding a viscosity above sqrt(min * max); the 6 rigid motions of each of the
+     rigid_max_inclusions largest form W, and the preconditioner becomes
+     z = W y + M^-1 (r - A W y), y = (W^T A W)^-1 W^T r (A-DEF2 around the V-cycle M^-1).
+     Set-up cost: 6 fine applies per inclusion.  Memory: A W on the nodes it reaches. */
+  int rigid_deflation;      /* 1: on, 0: off (default) */
+  double rigid_min_contrast; /* default 100 (>= 1) */
+  int rigid_max_inclusions;  /* default 128 (1..128) */
 } ptq2_mg_opts;
 
 enum { PTQ2_ETA_MEAN_ARITHMETIC = 0, PTQ2_ETA_MEAN_GEOMETRIC = 1, PTQ2_ETA_MEAN_HARMONIC = 2, PTQ2_ETA_MEAN_MAX = 3 };
@@ -56,6 +67,11 @@ void ptq2_mg_default_opts(ptq2_mg_opts *opts);
 int ptq2_mg_create(ptq2_mg **mg, ptq2_ctx *fine, const ptq2_mg_opts *opts);
 int ptq2_mg_destroy(ptq2_mg **mg);
 int ptq2_mg_levels(const ptq2_mg *mg, int *nlevels, int *m_per_level /* [nlevels] x,y,z triples */);
+
+/* the stiff inclusions of the two-level correction: their number, the rigid-motion columns
+   kept in (W^T A W)^-1, and, if node_label is not NULL, the inclusion of every fine node
+   ((2mx+1)(2my+1)(2mz+1) entries, -1 outside all; 0 inclusions when the option is off) */
+int ptq2_mg_rigid_info(const ptq2_mg *mg, int *ninclusions, int *ncolumns, int32_t *node_label);
 
 /* the Chebyshev interval [emin, emax] of every smoothed level (nlevels-1 entries each) */
 int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax);

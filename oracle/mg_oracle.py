@@ -267,3 +267,134 @@ class VCycle:
             corr[L.mask > 0] = 0.0
         x = x + corr
         return chebyshev(L, b, x, self.dinv[l], self.emin[l], self.emax[l], self.its, False)
+
+
+# ---------------------------------------------------------------------------------------
+# Rigid-body deflation of stiff inclusions (ptq2_mg_opts.rigid_deflation).  No counterpart
+# in the reference, whose velocity solve is PCMG + FGMRES only; this restates the library's
+# two-level correction so that its GPU form can be checked.
+#
+# Stiff elements: those with a quadrature viscosity above sqrt(min * max) of all quadrature
+# values, when max >= min_contrast * min (a rigid motion of an inclusion must cover every
+# stiff quadrature point, so an element counts as soon as one of its points does).  Inclusions: their
+# connected sets (elements sharing at least one node), numbered by their first element; with
+# more than max_incl the ones with the most elements are kept (ties: the earlier one).  Each
+# kept inclusion contributes 6 columns to W, on the nodes of its elements: the translations
+# e_a and the rotations e_a x (x - c) / L (c the mean of the nodes' coordinates, L their
+# RMS distance to it), Dirichlet dofs zeroed.  With E = W^T A W, the correction of a
+# preconditioner M^-1 is (Tang et al.'s A-DEF2)
+#   z = W E^-1 W^T r + M^-1 (r - A W E^-1 W^T r),
+# E^-1 over the columns kept by a Cholesky factorisation that skips a column whose pivot is
+# <= rel * E_jj (a fully constrained or dependent motion).
+# ---------------------------------------------------------------------------------------
+def stiff_inclusions(mx, my, mz, eta, min_contrast=100.0, max_incl=128):
+    """element labels (-1 outside) and the number of inclusions, by a breadth-first walk over
+    the 26 neighbours of each element."""
+    e = np.asarray(eta, dtype=np.float64).reshape(-1, 27)
+    ne = mx * my * mz
+    lab = np.full(ne, -1, dtype=np.int64)
+    lo, hi = e.min(), e.max()
+    if not hi >= min_contrast * lo:
+        return lab.astype(np.int32), 0
+    stiff = e.max(axis=1) > np.sqrt(lo * hi)
+    first, size = [], []
+    for e0 in range(ne):
+        if not stiff[e0] or lab[e0] >= 0:
+            continue
+        c = len(first)
+        first.append(e0)
+        lab[e0] = c
+        stack, cnt = [e0], 0
+        while stack:
+            e = stack.pop()
+            cnt += 1
+            ex, ey, ez = e % mx, (e // mx) % my, e // (mx * my)
+            for dz in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dx in (-1, 0, 1):
+                        x, y, z = ex + dx, ey + dy, ez + dz
+                        if 0 <= x < mx and 0 <= y < my and 0 <= z < mz:
+                            f = x + mx * (y + my * z)
+                            if stiff[f] and lab[f] < 0:
+                                lab[f] = c
+                                stack.append(f)
+        size.append(cnt)
+    nc = len(first)
+    if nc > max_incl:
+        keep = sorted(sorted(range(nc), key=lambda c: (-size[c], first[c]))[:max_incl])
+        new = np.full(nc, -1, dtype=np.int64)
+        new[keep] = np.arange(len(keep))
+        lab = np.where(lab >= 0, new[np.maximum(lab, 0)], -1)
+        nc = max_incl
+    return lab.astype(np.int32), nc
+
+
+def inclusion_node_labels(mx, my, mz, elem_label):
+    el = O.elnidx(mx, my, mz).reshape(-1, 27)
+    nl = np.full(O.nnodes(mx, my, mz), -1, dtype=np.int32)
+    st = elem_label >= 0
+    nl[el[st].reshape(-1)] = np.repeat(elem_label[st], 27)
+    return nl
+
+
+def rigid_basis(coords, mask, node_label, nc):
+    """W (3N x 6 nc): per inclusion the translations, then the rotations about x, y, z."""
+    X = np.asarray(coords).reshape(-1, 3)
+    W = np.zeros((X.size, 6 * nc))
+    for c in range(nc):
+        idx = np.nonzero(node_label == c)[0]
+        d = X[idx] - X[idx].mean(axis=0)
+        d = d / np.sqrt((d * d).sum() / idx.size)
+        for a in range(3):
+            W[3 * idx + a, 6 * c + a] = 1.0
+            e = np.zeros(3)
+            e[a] = 1.0
+            W[(3 * idx[:, None] + np.arange(3)).reshape(-1), 6 * c + 3 + a] = np.cross(e, d).reshape(-1)
+    if mask is not None:
+        W[np.asarray(mask) > 0, :] = 0.0
+    return W
+
+
+def drop_cholesky_keep(E, rel=1e-12):
+    """columns kept by a left-looking Cholesky factorisation of E that skips a column whose
+    pivot is not above rel * E_jj"""
+    n = E.shape[0]
+    keep, L = [], np.zeros((n, n))
+    for j in range(n):
+        if not E[j, j] > 0.0:
+            continue
+        k = len(keep)
+        l = np.zeros(k)
+        for i in range(k):
+            l[i] = (E[keep[i], j] - L[i, :i] @ l[:i]) / L[i, i]
+        piv = E[j, j] - l @ l
+        if not piv > rel * E[j, j]:
+            continue
+        L[k, :k] = l
+        L[k, k] = np.sqrt(piv)
+        keep.append(j)
+    return keep
+
+
+class RigidDeflation:
+    """A-DEF2 correction around a preconditioner, for level L (MatMult semantics)."""
+
+    def __init__(self, L, min_contrast=100.0, max_incl=128, rel=1e-12):
+        self.elem_label, self.nc = stiff_inclusions(L.mx, L.my, L.mz, L.eta, min_contrast, max_incl)
+        self.node_label = inclusion_node_labels(L.mx, L.my, L.mz, self.elem_label)
+        self.W = rigid_basis(L.coords, L.mask, self.node_label, self.nc)
+        self.AW = np.stack([L.matmult(self.W[:, j]) for j in range(self.W.shape[1])], axis=1) \
+            if self.nc else np.zeros((L.n, 0))
+        E = self.W.T @ self.AW
+        E = 0.5 * (E + E.T)
+        self.keep = drop_cholesky_keep(E, rel)
+        self.Einv = np.zeros_like(E)
+        if self.keep:
+            k = np.array(self.keep)
+            self.Einv[np.ix_(k, k)] = np.linalg.inv(E[np.ix_(k, k)])
+
+    def __call__(self, M, r):
+        if self.nc == 0:
+            return M(r)
+        y = self.Einv @ (self.W.T @ r)
+        return self.W @ y + M(r - self.AW @ y)

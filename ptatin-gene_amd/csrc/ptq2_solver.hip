@@ -483,6 +483,162 @@ __global__ void k_recip(int64_t n, const double *__restrict__ d, T *__restrict__
 }
 
 // ------------------------------------------------------------------- levels
+
+// ---------------------------------------------------------------- rigid-body deflation
+// Opt-in two-level correction for stiff inclusions (ptq2_mg_opts.rigid_deflation; the
+// reference's velocity solve is PCMG + FGMRES only and has no counterpart).  An inclusion is a
+// connected set of elements (sharing a node) that hold a quadrature viscosity above
+// sqrt(min * max) of the field; its 6 rigid motions -- translations e_a and rotations
+// e_a x (x - c) / L on the nodes of its elements (c the mean of those nodes, L their RMS
+// distance to it), Dirichlet dofs zeroed -- are columns of W.  Around the V-cycle M^-1 the
+// preconditioner becomes (A-DEF2 of Tang, Nabben, Vuik and Erlangga, J. Sci. Comput. 39, 2009)
+//   z = W y + M^-1 (r - A W y),   y = E^-1 W^T r,   E = W^T A W.
+// A W is kept on the nodes it reaches (the inclusion's elements and their neighbours), so the
+// correction needs no operator apply: W^T r as chunked moments over the inclusions' nodes,
+// y by a dense product with E^-1, then two updates over those nodes.  Fixed summation order
+// throughout.
+constexpr int RG_CH = 1024;     // inclusion nodes per moment block
+constexpr int RG_MAXINC = 128;  // inclusions (6 columns each)
+
+struct RgChunk {
+  int comp, begin, end, pad;
+};
+
+// part[6 q + k]: moment k of x over chunk q (k < 3: sum of x_a; 3..5: sum of d x x, d the
+// scaled offset from the inclusion's centre), Dirichlet dofs excluded
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_rg_moments(const T *__restrict__ x, const uint8_t *__restrict__ mask,
+                                                   const double *__restrict__ coords, const int *__restrict__ nodes,
+                                                   const RgChunk *__restrict__ chunk, const double4 *__restrict__ cent,
+                                                   double *__restrict__ part) {
+  __shared__ double red[6][TPB / 64];
+  const RgChunk ch = chunk[blockIdx.x];
+  const double4 g = cent[ch.comp];
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int i = ch.begin + (int)threadIdx.x; i < ch.end; i += TPB) {
+    const int64_t o = 3 * (int64_t)nodes[i];
+    double r[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) r[a] = (mask && mask[o + a]) ? 0.0 : (double)x[o + a];
+    const double d0 = (coords[o] - g.x) * g.w, d1 = (coords[o + 1] - g.y) * g.w, d2 = (coords[o + 2] - g.z) * g.w;
+    s[0] += r[0];
+    s[1] += r[1];
+    s[2] += r[2];
+    s[3] += d1 * r[2] - d2 * r[1];
+    s[4] += d2 * r[0] - d0 * r[2];
+    s[5] += d0 * r[1] - d1 * r[0];
+  }
+  block_partials<6>(s, red);
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int m = threadIdx.x;
+    part[6 * (int64_t)blockIdx.x + m] = red[m][0] + red[m][1] + red[m][2] + red[m][3];
+  }
+}
+
+__device__ __forceinline__ double rg_msum(int t, const int *__restrict__ cchunk, const double *__restrict__ part) {
+  const int c = t / 6, k = t % 6;
+  double s = 0.0;
+  for (int q = cchunk[c]; q < cchunk[c + 1]; q++) s += part[6 * (int64_t)q + k];
+  return s;
+}
+
+// out[t] = moment t of the last k_rg_moments pass (set-up: a column of E)
+__global__ void k_rg_msum(int n6, const int *__restrict__ cchunk, const double *__restrict__ part,
+                          double *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n6) out[t] = rg_msum(t, cchunk, part);
+}
+
+// y = E^-1 m (m: the chunk moments summed per inclusion in chunk order); 4 rows per block
+__global__ __launch_bounds__(256) void k_rg_coarse(int n6, const int *__restrict__ cchunk,
+                                                  const double *__restrict__ part, const double *__restrict__ einv,
+                                                  double *__restrict__ y) {
+  __shared__ double m[6 * RG_MAXINC];
+  for (int t = threadIdx.x; t < n6; t += 256) m[t] = rg_msum(t, cchunk, part);
+  __syncthreads();
+  const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n6) return;  // whole waves leave together
+  double s = 0.0;
+  for (int j = lane; j < n6; j += 64) s += einv[(int64_t)row * n6 + j] * m[j];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) y[row] = s;
+}
+
+// b -= (A W) y on the nodes A W reaches
+template <typename T>
+__global__ void k_rg_subtract(int64_t nd, const int *__restrict__ dnode, const int *__restrict__ dptr,
+                              const int *__restrict__ ecomp, const double *__restrict__ aw,
+                              const double *__restrict__ y, T *__restrict__ b) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nd; t += (int64_t)gridDim.x * blockDim.x) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int e = dptr[t]; e < dptr[t + 1]; e++) {
+      const double *yc = y + 6 * ecomp[e];
+      const double *w = aw + 18 * (int64_t)e;
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        a0 += w[3 * k] * yc[k];
+        a1 += w[3 * k + 1] * yc[k];
+        a2 += w[3 * k + 2] * yc[k];
+      }
+    }
+    const int64_t o = 3 * (int64_t)dnode[t];
+    b[o] = (T)((double)b[o] - a0);
+    b[o + 1] = (T)((double)b[o + 1] - a1);
+    b[o + 2] = (T)((double)b[o + 2] - a2);
+  }
+}
+
+// x += W y on the inclusions' nodes (Dirichlet dofs untouched)
+template <typename T>
+__global__ __launch_bounds__(TPB) void k_rg_expand(const uint8_t *__restrict__ mask, const double *__restrict__ coords,
+                                                  const int *__restrict__ nodes, const RgChunk *__restrict__ chunk,
+                                                  const double4 *__restrict__ cent, const double *__restrict__ y,
+                                                  T *__restrict__ x) {
+  const RgChunk ch = chunk[blockIdx.x];
+  const double4 g = cent[ch.comp];
+  const double *yc = y + 6 * ch.comp;
+  const double t0 = yc[0], t1 = yc[1], t2 = yc[2], w0 = yc[3], w1 = yc[4], w2 = yc[5];
+  for (int i = ch.begin + (int)threadIdx.x; i < ch.end; i += TPB) {
+    const int64_t o = 3 * (int64_t)nodes[i];
+    const double d0 = (coords[o] - g.x) * g.w, d1 = (coords[o + 1] - g.y) * g.w, d2 = (coords[o + 2] - g.z) * g.w;
+    const double u[3] = {t0 + w1 * d2 - w2 * d1, t1 + w2 * d0 - w0 * d2, t2 + w0 * d1 - w1 * d0};
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+      if (!(mask && mask[o + a])) x[o + a] = (T)((double)x[o + a] + u[a]);
+  }
+}
+
+// the range of the quadrature viscosity of every element (natural element order)
+__global__ void k_eta_elem_range(Box f, const double *__restrict__ eta_bm, double *__restrict__ emax,
+                                 double *__restrict__ emin) {
+  const int64_t ne = (int64_t)f.mx * f.my * f.mz;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += (int64_t)gridDim.x * blockDim.x) {
+    const int ex = (int)(e % f.mx), ey = (int)((e / f.mx) % f.my), ez = (int)(e / ((int64_t)f.mx * f.my));
+    const int64_t lin = (ex >> 2) + (int64_t)f.nbx * ((ey >> 2) + (int64_t)f.nby * (ez >> 2));
+    const double *src = eta_bm + lin * 27 * ptq2::BEL + ptq2::elem_lane(ex & 3, ey & 3, ez & 3);
+    double hi = src[0], lo = src[0];
+    for (int q = 1; q < 27; q++) {
+      hi = fmax(hi, src[q * ptq2::BEL]);
+      lo = fmin(lo, src[q * ptq2::BEL]);
+    }
+    emax[e] = hi;
+    emin[e] = lo;
+  }
+}
+
+// aw[18 e + 3 k + a] = v[3 node + a] for the entries e of inclusion c (column k of its A W)
+__global__ void k_rg_gather(int64_t nd, const int *__restrict__ dnode, const int *__restrict__ dptr,
+                            const int *__restrict__ ecomp, int c, int k, const double *__restrict__ v,
+                            double *__restrict__ aw) {
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nd; t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = 3 * (int64_t)dnode[t];
+    for (int e = dptr[t]; e < dptr[t + 1]; e++)
+      if (ecomp[e] == c)
+        for (int a = 0; a < 3; a++) aw[18 * (int64_t)e + 3 * k + a] = v[o + a];
+  }
+}
+
 struct DevBuf {
   void *p = nullptr;
   ~DevBuf() { if (p) (void)hipFree(p); }
@@ -514,6 +670,14 @@ struct ptq2_mg {
   std::vector<double> hist;
   std::string last;
   void *fine_stream0 = nullptr;  // the fine context's stream before the hierarchy was built
+  // rigid-body deflation of stiff inclusions (rg_nc == 0: off or none found)
+  int rg_nc = 0, rg_n6 = 0, rg_kept = 0, rg_nchunk = 0;
+  int64_t rg_nd = 0;
+  std::vector<int> rg_label;  // inclusion of every fine node (-1: none)
+  int *rg_nodes = nullptr, *rg_cchunk = nullptr, *rg_dnode = nullptr, *rg_dptr = nullptr, *rg_ecomp = nullptr;
+  RgChunk *rg_chunk = nullptr;
+  double4 *rg_cent = nullptr;
+  double *rg_part = nullptr, *rg_y = nullptr, *rg_einv = nullptr, *rg_aw = nullptr;
 };
 
 namespace {
@@ -628,19 +792,47 @@ int vcycle_level(ptq2_mg *mg, int l) {
   return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle launch");
 }
 
+// A-DEF2 steps around the V-cycle on the level-0 vectors: before it b -= A W y with
+// y = E^-1 W^T b, after it x += W y
+template <typename T>
+void rigid_pre(ptq2_mg *mg, T *b) {
+  if (!mg->rg_nc) return;
+  const ptq2_ctx *f = mg->ctx[0];
+  hipLaunchKernelGGL(k_rg_moments<T>, dim3(mg->rg_nchunk), dim3(TPB), 0, mg->stream, (const T *)b,
+                     (const uint8_t *)f->mask, (const double *)f->coords, (const int *)mg->rg_nodes,
+                     (const RgChunk *)mg->rg_chunk, (const double4 *)mg->rg_cent, mg->rg_part);
+  hipLaunchKernelGGL(k_rg_coarse, dim3((mg->rg_n6 + 3) / 4), dim3(256), 0, mg->stream, mg->rg_n6,
+                     (const int *)mg->rg_cchunk, (const double *)mg->rg_part, (const double *)mg->rg_einv, mg->rg_y);
+  hipLaunchKernelGGL(k_rg_subtract<T>, dim3(nblk(mg->rg_nd)), dim3(TPB), 0, mg->stream, mg->rg_nd,
+                     (const int *)mg->rg_dnode, (const int *)mg->rg_dptr, (const int *)mg->rg_ecomp,
+                     (const double *)mg->rg_aw, (const double *)mg->rg_y, b);
+}
+template <typename T>
+void rigid_post(ptq2_mg *mg, T *x) {
+  if (!mg->rg_nc) return;
+  const ptq2_ctx *f = mg->ctx[0];
+  hipLaunchKernelGGL(k_rg_expand<T>, dim3(mg->rg_nchunk), dim3(TPB), 0, mg->stream, (const uint8_t *)f->mask,
+                     (const double *)f->coords, (const int *)mg->rg_nodes, (const RgChunk *)mg->rg_chunk,
+                     (const double4 *)mg->rg_cent, (const double *)mg->rg_y, x);
+}
+
 // the work of one V-cycle: io->out = io->sout * M^-1 (io->sin * io->in), FP64 vectors
 int vcycle_body(ptq2_mg *mg) {
   const int64_t n = mg->n[0];
   if (mg->f32) {
     hipLaunchKernelGGL(k_vin<float>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
                        (float *)mg->vb[0]);
+    rigid_pre<float>(mg, (float *)mg->vb[0]);
     MGRC(vcycle_level<float>(mg, 0));
+    rigid_post<float>(mg, (float *)mg->vx[0]);
     hipLaunchKernelGGL(k_vout<float>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
                        (const float *)mg->vx[0]);
   } else {
     hipLaunchKernelGGL(k_vin<double>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
                        (double *)mg->vb[0]);
+    rigid_pre<double>(mg, (double *)mg->vb[0]);
     MGRC(vcycle_level<double>(mg, 0));
+    rigid_post<double>(mg, (double *)mg->vx[0]);
     hipLaunchKernelGGL(k_vout<double>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, (const VcIO *)mg->io, n,
                        (const double *)mg->vx[0]);
   }
@@ -850,6 +1042,246 @@ __global__ void k_negate(int64_t n, double *__restrict__ x) {
 }
 
 
+// Rigid-body deflation set-up on the fine level: find the inclusions, build the node lists,
+// then E = W^T A W and the A W entries column by column (one fine apply per column), and
+// E^-1 over the columns a left-looking Cholesky factorisation keeps (a column whose pivot
+// is not above rel * E_jj -- fully constrained or dependent -- gets a zero row and column).
+int rigid_setup(ptq2_mg *mg) {
+  ptq2_ctx *f = mg->ctx[0];
+  const int mx = f->box.mx, my = f->box.my;  // element layers in z: ne / (mx my)
+  const int NX = 2 * mx + 1, NY = 2 * my + 1;
+  const int64_t ne = f->nel, nn = f->nnodes, n = f->nvel;
+  std::vector<double> emx((size_t)ne), emn((size_t)ne);
+  {
+    double *d = nullptr;
+    MGRC(mg_alloc(mg, (void **)&d, sizeof(double) * 2 * (size_t)ne));
+    hipLaunchKernelGGL(k_eta_elem_range, dim3(nblk(ne)), dim3(TPB), 0, mg->stream, f->box, (const double *)f->eta_bm, d,
+                       d + ne);
+    hipError_t e1 = hipMemcpyAsync(emx.data(), d, sizeof(double) * ne, hipMemcpyDeviceToHost, mg->stream);
+    hipError_t e2 = hipMemcpyAsync(emn.data(), d + ne, sizeof(double) * ne, hipMemcpyDeviceToHost, mg->stream);
+    hipError_t e3 = hipStreamSynchronize(mg->stream);
+    (void)hipFree(d);
+    MGCHK(e1);
+    MGCHK(e2);
+    MGCHK(e3);
+  }
+  const double lo = *std::min_element(emn.begin(), emn.end()), hi = *std::max_element(emx.begin(), emx.end());
+  mg->rg_label.assign((size_t)nn, -1);
+  if (!(hi >= mg->opt.rigid_min_contrast * lo)) return PTQ2_OK;
+  const double tau = std::sqrt(lo * hi);
+  // connected stiff elements (26 neighbours) by union-find, numbered by their first element
+  std::vector<int64_t> par((size_t)ne, -1);
+  for (int64_t e = 0; e < ne; e++)
+    if (emx[e] > tau) par[e] = e;
+  auto find = [&](int64_t a) {
+    while (par[a] != a) {
+      par[a] = par[par[a]];
+      a = par[a];
+    }
+    return a;
+  };
+  for (int64_t e = 0; e < ne; e++) {
+    if (par[e] < 0) continue;
+    const int ex = (int)(e % mx), ey = (int)((e / mx) % my), ez = (int)(e / ((int64_t)mx * my));
+    for (int dz = -1; dz <= 0; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          if (dz == 0 && (dy > 0 || (dy == 0 && dx >= 0))) continue;  // the 13 earlier neighbours
+          const int x = ex + dx, y = ey + dy, z = ez + dz;
+          if (x < 0 || x >= mx || y < 0 || y >= my || z < 0) continue;
+          const int64_t g = x + (int64_t)mx * (y + (int64_t)my * z);
+          if (par[g] < 0) continue;
+          const int64_t ra = find(e), rb = find(g);
+          if (ra != rb) par[std::max(ra, rb)] = std::min(ra, rb);
+        }
+  }
+  std::vector<int> lab((size_t)ne, -1), size;
+  std::vector<int64_t> rid((size_t)ne, -1);
+  for (int64_t e = 0; e < ne; e++) {
+    if (par[e] < 0) continue;
+    const int64_t r = find(e);
+    if (rid[r] < 0) {
+      rid[r] = (int64_t)size.size();
+      size.push_back(0);
+    }
+    lab[e] = (int)rid[r];
+    size[lab[e]]++;
+  }
+  int nc = (int)size.size();
+  if (nc > mg->opt.rigid_max_inclusions) {
+    // keep the largest (ties: the earlier), renumbered in their original order
+    std::vector<int> ord(nc);
+    for (int c = 0; c < nc; c++) ord[c] = c;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return size[a] > size[b]; });
+    std::vector<int> keep(ord.begin(), ord.begin() + mg->opt.rigid_max_inclusions), map(nc, -1);
+    std::sort(keep.begin(), keep.end());
+    for (size_t k = 0; k < keep.size(); k++) map[keep[k]] = (int)k;
+    for (int64_t e = 0; e < ne; e++)
+      if (lab[e] >= 0) lab[e] = map[lab[e]];
+    nc = mg->opt.rigid_max_inclusions;
+  }
+  if (nc == 0) return PTQ2_OK;
+  auto enode = [&](int64_t e, int q) {
+    const int ex = (int)(e % mx), ey = (int)((e / mx) % my), ez = (int)(e / ((int64_t)mx * my));
+    return (int64_t)(2 * ex + q % 3) + (int64_t)NX * ((2 * ey + (q / 3) % 3) + (int64_t)NY * (2 * ez + q / 9));
+  };
+  for (int64_t e = 0; e < ne; e++)
+    if (lab[e] >= 0)
+      for (int q = 0; q < 27; q++) mg->rg_label[enode(e, q)] = lab[e];
+  // nodes of every inclusion in increasing order, and their chunks
+  std::vector<int> cptr(nc + 1, 0), nodes;
+  for (int64_t i = 0; i < nn; i++)
+    if (mg->rg_label[i] >= 0) cptr[mg->rg_label[i] + 1]++;
+  for (int c = 0; c < nc; c++) cptr[c + 1] += cptr[c];
+  nodes.resize(cptr[nc]);
+  {
+    std::vector<int> at(cptr.begin(), cptr.end() - 1);
+    for (int64_t i = 0; i < nn; i++)
+      if (mg->rg_label[i] >= 0) nodes[at[mg->rg_label[i]]++] = (int)i;
+  }
+  std::vector<RgChunk> chunks;
+  std::vector<int> cchunk(nc + 1, 0);
+  for (int c = 0; c < nc; c++) {
+    for (int b = cptr[c]; b < cptr[c + 1]; b += RG_CH) chunks.push_back({c, b, std::min(b + RG_CH, cptr[c + 1]), 0});
+    cchunk[c + 1] = (int)chunks.size();
+  }
+  // centre and RMS radius from the coordinates
+  std::vector<double> X((size_t)n);
+  MGCHK(hipMemcpy(X.data(), f->coords, sizeof(double) * n, hipMemcpyDeviceToHost));
+  std::vector<double4> cent(nc);
+  for (int c = 0; c < nc; c++) {
+    double s[3] = {0.0, 0.0, 0.0}, q = 0.0;
+    const int cnt = cptr[c + 1] - cptr[c];
+    for (int i = cptr[c]; i < cptr[c + 1]; i++)
+      for (int a = 0; a < 3; a++) s[a] += X[3 * (int64_t)nodes[i] + a];
+    for (int a = 0; a < 3; a++) s[a] /= cnt;
+    for (int i = cptr[c]; i < cptr[c + 1]; i++)
+      for (int a = 0; a < 3; a++) {
+        const double d = X[3 * (int64_t)nodes[i] + a] - s[a];
+        q += d * d;
+      }
+    cent[c] = make_double4(s[0], s[1], s[2], 1.0 / std::sqrt(q / cnt));
+  }
+  // (node, inclusion) pairs reached by A W: the nodes of every element touching an inclusion
+  std::vector<uint64_t> pr;
+  for (int64_t e = 0; e < ne; e++) {
+    int seen[27], ns = 0;
+    for (int q = 0; q < 27; q++) {
+      const int c = mg->rg_label[enode(e, q)];
+      if (c < 0) continue;
+      bool dup = false;
+      for (int k = 0; k < ns; k++) dup = dup || seen[k] == c;
+      if (!dup) seen[ns++] = c;
+    }
+    for (int k = 0; k < ns; k++)
+      for (int q = 0; q < 27; q++) pr.push_back(((uint64_t)enode(e, q) << 8) | (uint64_t)seen[k]);
+  }
+  std::sort(pr.begin(), pr.end());
+  pr.erase(std::unique(pr.begin(), pr.end()), pr.end());
+  std::vector<int> dnode, dptr, ecomp(pr.size());
+  for (size_t i = 0; i < pr.size(); i++) {
+    const int node = (int)(pr[i] >> 8);
+    if (dnode.empty() || dnode.back() != node) {
+      dnode.push_back(node);
+      dptr.push_back((int)i);
+    }
+    ecomp[i] = (int)(pr[i] & 0xFF);
+  }
+  dptr.push_back((int)pr.size());
+  const int n6 = 6 * nc;
+  mg->rg_nc = nc;
+  mg->rg_n6 = n6;
+  mg->rg_nchunk = (int)chunks.size();
+  mg->rg_nd = (int64_t)dnode.size();
+  auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+    MGRC(mg_alloc(mg, dst, bytes));
+    MGCHK(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+    return PTQ2_OK;
+  };
+  MGRC(up((void **)&mg->rg_nodes, nodes.data(), sizeof(int) * nodes.size()));
+  MGRC(up((void **)&mg->rg_chunk, chunks.data(), sizeof(RgChunk) * chunks.size()));
+  MGRC(up((void **)&mg->rg_cchunk, cchunk.data(), sizeof(int) * cchunk.size()));
+  MGRC(up((void **)&mg->rg_cent, cent.data(), sizeof(double4) * cent.size()));
+  MGRC(up((void **)&mg->rg_dnode, dnode.data(), sizeof(int) * dnode.size()));
+  MGRC(up((void **)&mg->rg_dptr, dptr.data(), sizeof(int) * dptr.size()));
+  MGRC(up((void **)&mg->rg_ecomp, ecomp.data(), sizeof(int) * ecomp.size()));
+  MGRC(mg_alloc(mg, (void **)&mg->rg_part, sizeof(double) * 6 * chunks.size()));
+  MGRC(mg_alloc(mg, (void **)&mg->rg_y, sizeof(double) * n6));
+  MGRC(mg_alloc(mg, (void **)&mg->rg_einv, sizeof(double) * n6 * n6));
+  MGRC(mg_alloc(mg, (void **)&mg->rg_aw, sizeof(double) * 18 * pr.size()));
+  // columns of E and of A W
+  double *w = nullptr, *v = nullptr;
+  MGRC(mg_alloc(mg, (void **)&w, sizeof(double) * n));
+  int rc = mg_alloc(mg, (void **)&v, sizeof(double) * n);
+  for (int j = 0; j < n6 && rc == PTQ2_OK; j++) {
+    hipLaunchKernelGGL(k_unit, dim3(nblk(n6)), dim3(TPB), 0, mg->stream, (int64_t)n6, (int64_t)j, mg->rg_y);
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, w, 0.0);
+    rigid_post<double>(mg, w);
+    rc = ptq2_a11_apply(f, w, v, PTQ2_DEVICE | PTQ2_MATMULT);
+    if (rc) break;
+    hipLaunchKernelGGL(k_rg_moments<double>, dim3(mg->rg_nchunk), dim3(TPB), 0, mg->stream, (const double *)v,
+                       (const uint8_t *)f->mask, (const double *)f->coords, (const int *)mg->rg_nodes,
+                       (const RgChunk *)mg->rg_chunk, (const double4 *)mg->rg_cent, mg->rg_part);
+    hipLaunchKernelGGL(k_rg_msum, dim3((n6 + 255) / 256), dim3(256), 0, mg->stream, n6, (const int *)mg->rg_cchunk,
+                       (const double *)mg->rg_part, mg->rg_einv + (int64_t)j * n6);
+    hipLaunchKernelGGL(k_rg_gather, dim3(nblk(mg->rg_nd)), dim3(TPB), 0, mg->stream, mg->rg_nd,
+                       (const int *)mg->rg_dnode, (const int *)mg->rg_dptr, (const int *)mg->rg_ecomp, j / 6, j % 6,
+                       (const double *)v, mg->rg_aw);
+    if (hipGetLastError() != hipSuccess) rc = mgfail(mg, PTQ2_ERR_DEVICE, "rigid deflation set-up launch");
+  }
+  std::vector<double> E((size_t)n6 * n6);
+  if (rc == PTQ2_OK &&
+      hipMemcpyAsync(E.data(), mg->rg_einv, sizeof(double) * E.size(), hipMemcpyDeviceToHost, mg->stream) != hipSuccess)
+    rc = mgfail(mg, PTQ2_ERR_DEVICE, "rigid deflation copy");
+  if (hipStreamSynchronize(mg->stream) != hipSuccess && rc == PTQ2_OK) rc = mgfail(mg, PTQ2_ERR_DEVICE, "rigid deflation");
+  (void)hipFree(w);
+  if (v) (void)hipFree(v);
+  MGRC(rc);
+  for (int i = 0; i < n6; i++)
+    for (int j = 0; j < i; j++) E[(size_t)i * n6 + j] = E[(size_t)j * n6 + i] = 0.5 * (E[(size_t)i * n6 + j] + E[(size_t)j * n6 + i]);
+  // left-looking Cholesky over the kept columns
+  const double rel = 1e-12;
+  std::vector<int> keep;
+  std::vector<double> L((size_t)n6 * n6, 0.0), l(n6);
+  for (int j = 0; j < n6; j++) {
+    const double ejj = E[(size_t)j * n6 + j];
+    if (!(ejj > 0.0)) continue;
+    const int k = (int)keep.size();
+    double piv = ejj;
+    for (int i = 0; i < k; i++) {
+      double t = E[(size_t)keep[i] * n6 + j];
+      for (int q = 0; q < i; q++) t -= L[(size_t)i * n6 + q] * l[q];
+      l[i] = t / L[(size_t)i * n6 + i];
+      piv -= l[i] * l[i];
+    }
+    if (!(piv > rel * ejj)) continue;
+    for (int q = 0; q < k; q++) L[(size_t)k * n6 + q] = l[q];
+    L[(size_t)k * n6 + k] = std::sqrt(piv);
+    keep.push_back(j);
+  }
+  const int nk = (int)keep.size();
+  // E_kept^-1 = L^-T L^-1, scattered into the full index range
+  std::vector<double> Li((size_t)nk * nk, 0.0), Ei((size_t)n6 * n6, 0.0);
+  for (int c = 0; c < nk; c++) {
+    Li[(size_t)c * nk + c] = 1.0 / L[(size_t)c * n6 + c];
+    for (int r = c + 1; r < nk; r++) {
+      double t = 0.0;
+      for (int q = c; q < r; q++) t += L[(size_t)r * n6 + q] * Li[(size_t)q * nk + c];
+      Li[(size_t)r * nk + c] = -t / L[(size_t)r * n6 + r];
+    }
+  }
+  for (int a = 0; a < nk; a++)
+    for (int b = 0; b <= a; b++) {
+      double t = 0.0;
+      for (int q = a; q < nk; q++) t += Li[(size_t)q * nk + a] * Li[(size_t)q * nk + b];
+      Ei[(size_t)keep[a] * n6 + keep[b]] = Ei[(size_t)keep[b] * n6 + keep[a]] = t;
+    }
+  mg->rg_kept = nk;
+  MGCHK(hipMemcpy(mg->rg_einv, Ei.data(), sizeof(double) * Ei.size(), hipMemcpyHostToDevice));
+  return PTQ2_OK;
+}
+
+
 // z = M^-1 (scale r): one V-cycle reading r and writing z directly (the input is read in full
 // by the graph's first kernel before its last kernel writes z, so r and z may alias)
 int vcycle_scaled(ptq2_mg *mg, const double *r, double scale, double *z) {
@@ -875,6 +1307,9 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->coarse_max_dofs = 20000;
   o->galerkin_coarse = 1;
   o->coarse_eta_mean = PTQ2_ETA_MEAN_ARITHMETIC;
+  o->rigid_deflation = 0;
+  o->rigid_min_contrast = 100.0;
+  o->rigid_max_inclusions = RG_MAXINC;
 }
 
 int ptq2_mg_destroy(ptq2_mg **pmg) {
@@ -893,7 +1328,9 @@ int ptq2_mg_destroy(ptq2_mg **pmg) {
   for (double *p : mg->Vk) (void)hipFree(p);
   for (double *p : mg->Zk) (void)hipFree(p);
   for (void *p : {(void *)mg->ainv, (void *)mg->io, (void *)mg->w, (void *)mg->part, (void *)mg->psum,
-                  (void *)mg->hdev, (void *)mg->Vptr, (void *)mg->Zptr})
+                  (void *)mg->hdev, (void *)mg->Vptr, (void *)mg->Zptr, (void *)mg->rg_nodes, (void *)mg->rg_cchunk,
+                  (void *)mg->rg_dnode, (void *)mg->rg_dptr, (void *)mg->rg_ecomp, (void *)mg->rg_chunk,
+                  (void *)mg->rg_cent, (void *)mg->rg_part, (void *)mg->rg_y, (void *)mg->rg_einv, (void *)mg->rg_aw})
     if (p) (void)hipFree(p);
   delete mg;
   *pmg = nullptr;
@@ -912,6 +1349,9 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   else ptq2_mg_default_opts(&o);
   if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0) ||
       o.coarse_eta_mean < PTQ2_ETA_MEAN_ARITHMETIC || o.coarse_eta_mean > PTQ2_ETA_MEAN_MAX)
+    return PTQ2_ERR_INVALID_ARG;
+  if (o.rigid_deflation &&
+      (!(o.rigid_min_contrast >= 1.0) || o.rigid_max_inclusions < 1 || o.rigid_max_inclusions > RG_MAXINC))
     return PTQ2_ERR_INVALID_ARG;
   for (int l = 1; l < o.nlevels; l++) {
     int f = 1 << l;
@@ -1126,6 +1566,8 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     (void)hipFree(P); (void)hipFree(C); (void)hipFree(Dm); (void)hipFree(err);
     if (herr) return mgfail(mg, PTQ2_ERR_MESH, "coarse operator is not positive definite");
   }
+  // ---- two-level correction for stiff inclusions
+  if (o.rigid_deflation) MGRC(rigid_setup(mg));
   // ---- graph of one V-cycle
   MGRC(mg_alloc(mg, (void **)&mg->io, sizeof(VcIO)));
   if (o.use_graph) {
@@ -1151,6 +1593,17 @@ int ptq2_mg_levels(const ptq2_mg *mg, int *nl, int *m) {
   if (!mg || !nl) return PTQ2_ERR_INVALID_ARG;
   *nl = mg->nl;
   if (m) std::memcpy(m, mg->m.data(), sizeof(int) * mg->m.size());
+  return PTQ2_OK;
+}
+
+int ptq2_mg_rigid_info(const ptq2_mg *mg, int *ninclusions, int *ncolumns, int32_t *node_label) {
+  if (!mg) return PTQ2_ERR_INVALID_ARG;
+  if (ninclusions) *ninclusions = mg->rg_nc;
+  if (ncolumns) *ncolumns = mg->rg_kept;
+  if (node_label) {
+    const int64_t nn = mg->ctx[0]->nnodes;
+    for (int64_t i = 0; i < nn; i++) node_label[i] = mg->rg_label.empty() ? -1 : mg->rg_label[i];
+  }
   return PTQ2_OK;
 }
 

@@ -17,7 +17,8 @@ class MGOpts(ctypes.Structure):
     _fields_ = [("nlevels", ctypes.c_int), ("smooth_its", ctypes.c_int), ("fp32_smoother", ctypes.c_int),
                 ("emin_frac", ctypes.c_double), ("emax_frac", ctypes.c_double), ("esteig_its", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int), ("galerkin_coarse", ctypes.c_int),
-                ("coarse_eta_mean", ctypes.c_int)]
+                ("coarse_eta_mean", ctypes.c_int), ("rigid_deflation", ctypes.c_int),
+                ("rigid_min_contrast", ctypes.c_double), ("rigid_max_inclusions", ctypes.c_int)]
 
 ETA_MEANS = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "max": 3}
 
@@ -28,6 +29,8 @@ _SOLVER_SIGS = {
     "ptq2_mg_destroy": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "ptq2_mg_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
     "ptq2_mg_eigs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "ptq2_mg_rigid_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.c_void_p]),
     "ptq2_mg_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "ptq2_mg_vcycle": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "ptq2_mg_solve": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
@@ -60,7 +63,8 @@ class ViscousSolver:
     """GMG V-cycle (hipGraph) + FGMRES for MatMult_MFStokes_A11 x = b on one GPU."""
 
     def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000,
-                 galerkin_coarse=True, eta_mean="arithmetic", emin_frac=None, emax_frac=None):
+                 galerkin_coarse=True, eta_mean="arithmetic", emin_frac=None, emax_frac=None, rigid_deflation=False,
+                 rigid_min_contrast=None, rigid_max_inclusions=None):
         L = _sl()
         o = default_opts()
         o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
@@ -73,6 +77,11 @@ class ViscousSolver:
             o.emin_frac = emin_frac
         if emax_frac is not None:
             o.emax_frac = emax_frac
+        o.rigid_deflation = 1 if rigid_deflation else 0
+        if rigid_min_contrast is not None:
+            o.rigid_min_contrast = rigid_min_contrast
+        if rigid_max_inclusions is not None:
+            o.rigid_max_inclusions = rigid_max_inclusions
         self.ctx = ctx  # keep the fine context alive
         h = ctypes.c_void_p()
         check(L.ptq2_mg_create(ctypes.byref(h), ctx.handle, ctypes.byref(o)), "ptq2_mg_create", ctx.handle)
@@ -91,6 +100,18 @@ class ViscousSolver:
         hi = np.zeros(self.nlevels - 1)
         check(_sl().ptq2_mg_eigs(self._h, lo.ctypes.data, hi.ctypes.data), "ptq2_mg_eigs")
         return lo, hi
+
+    def rigid_info(self, labels=False):
+        """(inclusions, kept rigid-motion columns[, per-node inclusion labels]) of the
+        two-level correction for stiff inclusions"""
+        ni, nk = ctypes.c_int(), ctypes.c_int()
+        lab = None
+        if labels:
+            mx, my, mz = self.levels()[0]
+            lab = np.zeros((2 * mx + 1) * (2 * my + 1) * (2 * mz + 1), dtype=np.int32)
+        check(_sl().ptq2_mg_rigid_info(self._h, ctypes.byref(ni), ctypes.byref(nk),
+                                       None if lab is None else lab.ctypes.data), "ptq2_mg_rigid_info")
+        return (ni.value, nk.value, lab) if labels else (ni.value, nk.value)
 
     def _stream(self, t):
         """Order the solver on torch's current stream (the null stream by default)."""

@@ -146,3 +146,54 @@ def test_galerkin_coarse_operator_is_rap():
     # the restriction is the transpose of the prolongation
     r = np.random.default_rng(0).standard_normal(nf)
     assert np.allclose(G.restrict(C.mx, C.my, C.mz, r), (np.stack([G.prolong(C.mx, C.my, C.mz, np.eye(nc)[j]) for j in range(nc)], axis=1)).T @ r)
+
+
+def test_stiff_inclusions_match_connected_component_labelling():
+    """Inclusions of the rigid-body correction = scipy's 26-connected labelling of the elements
+    with a stiff quadrature point (numbered in raster order), and the size cap keeps the largest."""
+    ndimage = __import__("pytest").importorskip("scipy.ndimage")
+    rng = np.random.default_rng(7)
+    for _ in range(12):
+        mx, my, mz = (int(v) for v in rng.integers(2, 9, 3))
+        eta = np.ones((mx * my * mz, 27))
+        hit = rng.random(eta.shape) < 0.01  # single stiff points make an element stiff
+        eta[hit] = 1e4
+        lab, nc = G.stiff_inclusions(mx, my, mz, eta.reshape(-1))
+        ref, nref = ndimage.label(hit.any(axis=1).reshape(mz, my, mx), structure=np.ones((3, 3, 3)))
+        assert nc == nref and np.array_equal(lab, ref.reshape(-1) - 1)
+        if nc >= 2:
+            sizes = np.bincount(lab[lab >= 0])
+            lab1, n1 = G.stiff_inclusions(mx, my, mz, eta.reshape(-1), max_incl=1)
+            big = int(np.argmax(sizes))  # first of the largest
+            assert n1 == 1 and np.array_equal(lab1 >= 0, lab == big)
+    # below the contrast threshold there is nothing to deflate
+    assert G.stiff_inclusions(2, 2, 2, np.full(8 * 27, 3.0))[1] == 0
+
+
+def test_rigid_motions_have_no_strain_and_adef2_reproduces_them():
+    """With every element stiff, W holds the 6 rigid motions of the whole mesh: A W = 0 without
+    BCs (isoparametric Q2 reproduces linear fields; jittered mesh).  With a Dirichlet mask and
+    one inclusion, the A-DEF2 correction maps r = A W c back to W c whatever M is, and a
+    dependent column is dropped by the factorisation."""
+    m = 3
+    c = M.jitter(m, m, m, M.q2_uniform(m, m, m), 0.1, 1.0 / (2 * m), 1)
+    eta = np.full((m ** 3, 27), 1e4)
+    eta[0, 0] = 1.0
+    L = G.Level(m, m, m, c, None, eta.reshape(-1))
+    D = G.RigidDeflation(L)
+    assert D.nc == 1 and (D.node_label == 0).all()
+    assert np.abs(D.AW).max() <= 1e-9 * np.abs(D.W).max() * 1e4
+    # translations and rotations are independent and of unit RMS size
+    assert np.linalg.matrix_rank(D.W) == 6
+    mask = M.bc_mask(m, m, m, "freeslip")
+    eta = np.ones((m ** 3, 27))
+    eta[13] = 1e3  # the centre element
+    L = G.Level(m, m, m, c, mask, eta.reshape(-1))
+    D = G.RigidDeflation(L)
+    assert D.nc == 1 and len(D.keep) == 6
+    coef = M.uniform(9, 6, -1, 1)
+    z = D(lambda r: np.sin(r), D.AW @ coef)  # any M: its input is zero
+    assert np.allclose(z, D.W @ coef, rtol=0, atol=1e-12)
+    Wd = np.concatenate([D.W, D.W[:, :1]], axis=1)
+    E = Wd.T @ np.stack([L.matmult(Wd[:, j]) for j in range(7)], axis=1)
+    assert G.drop_cholesky_keep(0.5 * (E + E.T)) == [0, 1, 2, 3, 4, 5]
