@@ -193,6 +193,8 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
         out[key + "_iterations"] = info["iterations"]
         out[key + "_rel_residual"] = info["rel"]
         out[key + "_setup_s"] = round(setup, 3)
+        if rigid:
+            out[key + "_inclusions_columns"] = list(S.rigid_info())
         print("%s smoother: %d its, rel %.3e, %.3f s" % (key, info["iterations"], info["rel"], el),
               file=sys.stderr, flush=True)
         S.destroy()
