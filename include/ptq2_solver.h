@@ -40,6 +40,10 @@ typedef struct {
   int coarse_eta_mean;  /* PTQ2_ETA_MEAN_*: how a coarse quadrature point combines the fine
                            quadrature viscosities of the child elements it touches (mean or
                            maximum) */
+  int fp32_basis;       /* 1: FGMRES keeps its finished Arnoldi vectors in FP32 (candidates,
+                           Z vectors, reductions and the true residual stay FP64): half the
+                           Gram-Schmidt traffic; restarts recompute the FP64 residual.  0
+                           (default): FP64 basis */
   /* Two-level correction for stiff inclusions (no counterpart in the reference, whose
      velocity solve is PCMG + FGMRES only; for high viscosity contrasts, where the rigid
      motions of the inclusions stall the Krylov solve).  When the quadrature viscosity spans
@@ -63,7 +67,8 @@ void ptq2_mg_default_opts(ptq2_mg_opts *opts);
 /* Build the hierarchy under a set-up context (mesh with BC mask + viscosity): coarse
    meshes by coordinate injection, BC masks by injection, quadrature-point viscosity
    by averaging over the 8 children of each coarse element, Jacobi diagonals,
-   eigen-estimates, the dense FP64 inverse of the coarsest operator and the graph. */
+   eigen-estimates, the dense FP64 inverse of the coarsest operator, with rigid_deflation the
+   stiff-inclusion correction, and the graph. */
 int ptq2_mg_create(ptq2_mg **mg, ptq2_ctx *fine, const ptq2_mg_opts *opts);
 int ptq2_mg_destroy(ptq2_mg **mg);
 int ptq2_mg_levels(const ptq2_mg *mg, int *nlevels, int *m_per_level /* [nlevels] x,y,z triples */);

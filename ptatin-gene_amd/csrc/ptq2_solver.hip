@@ -236,7 +236,8 @@ __device__ __forceinline__ void block_partials(double (&s)[NR], double (*red)[TP
 
 // part[v * MDB + block]: partial of V[v].w for v < nv; with self, part[nv * MDB + block]:
 // partial of w.w (taken in the first pass over w)
-__global__ __launch_bounds__(TPB) void k_mdot(int64_t n, const double *const *__restrict__ V, int nv,
+template <typename VT>
+__global__ __launch_bounds__(TPB) void k_mdot(int64_t n, const VT *const *__restrict__ V, int nv,
                                               const double *__restrict__ w, int self, double *__restrict__ part) {
   __shared__ double red[MD + 1][TPB / 64];
   const int64_t stride = (int64_t)MDB * TPB;
@@ -247,11 +248,11 @@ __global__ __launch_bounds__(TPB) void k_mdot(int64_t n, const double *const *__
     const int nm = max(0, min(MD, nv - v0));
     const bool ws = self && gi == 0;
     double s[MD + 1];
-    const double *vp[MD];
+    const VT *vp[MD];
 #pragma unroll
     for (int m = 0; m <= MD; m++) s[m] = 0.0;
 #pragma unroll
-    for (int m = 0; m < MD; m++) vp[m] = m < nm ? V[v0 + m] : w;
+    for (int m = 0; m < MD; m++) vp[m] = m < nm ? V[v0 + m] : nullptr;  // read only for m < nm
     int64_t i = i0;
     for (; i + (MDU - 1) * stride < n; i += MDU * stride) {
       double wv[MDU];
@@ -262,7 +263,7 @@ __global__ __launch_bounds__(TPB) void k_mdot(int64_t n, const double *const *__
         if (m < nm) {
           double x[MDU];
 #pragma unroll
-          for (int u = 0; u < MDU; u++) x[u] = vp[m][i + u * stride];
+          for (int u = 0; u < MDU; u++) x[u] = (double)vp[m][i + u * stride];
 #pragma unroll
           for (int u = 0; u < MDU; u++) s[m] = fma(x[u], wv[u], s[m]);
         }
@@ -274,7 +275,7 @@ __global__ __launch_bounds__(TPB) void k_mdot(int64_t n, const double *const *__
       const double wi = w[i];
 #pragma unroll
       for (int m = 0; m < MD; m++)
-        if (m < nm) s[m] = fma(vp[m][i], wi, s[m]);
+        if (m < nm) s[m] = fma((double)vp[m][i], wi, s[m]);
       if (ws) s[MD] = fma(wi, wi, s[MD]);
     }
     block_partials<MD + 1>(s, red);
@@ -300,14 +301,15 @@ __global__ __launch_bounds__(TPB) void k_sum_parts(const double *__restrict__ pa
 }
 
 // w -= sum_v h[v] V[v], and part[block]: partial of the updated w.w
-__global__ __launch_bounds__(TPB) void k_maxpy_norm(int64_t n, const double *const *__restrict__ V, int nv,
+template <typename VT>
+__global__ __launch_bounds__(TPB) void k_maxpy_norm(int64_t n, const VT *const *__restrict__ V, int nv,
                                                     const double *__restrict__ h, double *__restrict__ w,
                                                     double *__restrict__ part) {
   __shared__ double red[1][TPB / 64];
   double acc[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)MDB * TPB) {
     double s = w[i];
-    for (int v = 0; v < nv; v++) s -= h[v] * V[v][i];
+    for (int v = 0; v < nv; v++) s -= h[v] * (double)V[v][i];
     w[i] = s;
     acc[0] = fma(s, s, acc[0]);
   }
@@ -322,7 +324,8 @@ __global__ __launch_bounds__(TPB) void k_maxpy_norm(int64_t n, const double *con
 // Pass 1, part rows: 2v, 2v+1 -> V[v].u', V[v].w (v < nv); 2nv, 2nv+1, 2nv+2 -> u'.w, u'.u',
 // w.w, with u' the stored (unscaled) candidate.  MD2 basis vectors per pass over u' and w.
 constexpr int MD2 = 16;
-__global__ __launch_bounds__(TPB) void k_mdot2(int64_t n, const double *const *__restrict__ V, int nv,
+template <typename VT>
+__global__ __launch_bounds__(TPB) void k_mdot2(int64_t n, const VT *const *__restrict__ V, int nv,
                                                const double *__restrict__ u, const double *__restrict__ w,
                                                double *__restrict__ part) {
   constexpr int NR = 2 * MD2 + 3;
@@ -335,11 +338,11 @@ __global__ __launch_bounds__(TPB) void k_mdot2(int64_t n, const double *const *_
     const int nm = max(0, min(MD2, nv - v0));
     const bool first = gi == 0;
     double s[NR];
-    const double *vp[MD2];
+    const VT *vp[MD2];
 #pragma unroll
     for (int m = 0; m < NR; m++) s[m] = 0.0;
 #pragma unroll
-    for (int m = 0; m < MD2; m++) vp[m] = m < nm ? V[v0 + m] : u;
+    for (int m = 0; m < MD2; m++) vp[m] = m < nm ? V[v0 + m] : nullptr;  // read only for m < nm
     int64_t i = i0;
     for (; i + (MDU - 1) * stride < n; i += MDU * stride) {
       double uv[MDU], wv[MDU];
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(TPB) void k_mdot2(int64_t n, const double *const *_
         if (m < nm) {
           double x[MDU];
 #pragma unroll
-          for (int q = 0; q < MDU; q++) x[q] = vp[m][i + q * stride];
+          for (int q = 0; q < MDU; q++) x[q] = (double)vp[m][i + q * stride];
 #pragma unroll
           for (int q = 0; q < MDU; q++) {
             s[2 * m] = fma(x[q], uv[q], s[2 * m]);
@@ -373,7 +376,7 @@ __global__ __launch_bounds__(TPB) void k_mdot2(int64_t n, const double *const *_
 #pragma unroll
       for (int m = 0; m < MD2; m++)
         if (m < nm) {
-          const double x = vp[m][i];
+          const double x = (double)vp[m][i];
           s[2 * m] = fma(x, ui, s[2 * m]);
           s[2 * m + 1] = fma(x, wi, s[2 * m + 1]);
         }
@@ -395,26 +398,169 @@ __global__ __launch_bounds__(TPB) void k_mdot2(int64_t n, const double *const *_
 }
 
 // Pass 2: with uu = su * u' and the coefficient table ab = [a (nv), b (nv)]:
-//   u  <- (uu - sum_v a[v] V[v]) * ia      (the finished basis vector, written over u')
+//   vo <- (uu - sum_v a[v] V[v]) * ia      (the finished basis vector; vo may be u' itself)
 //   wo <- w - sum_v b[v] V[v] - gam * uu   (the next candidate, unnormalised)
 // and part[block]: partial of wo.wo.
-__global__ __launch_bounds__(TPB) void k_dcgs_update(int64_t n, const double *const *__restrict__ V, int nv,
+template <typename VT>
+__global__ __launch_bounds__(TPB) void k_dcgs_update(int64_t n, const VT *const *__restrict__ V, int nv,
                                                      const double *__restrict__ ab, double su, double ia, double gam,
-                                                     double *__restrict__ u, const double *__restrict__ w,
+                                                     const double *u, VT *vo, const double *__restrict__ w,
                                                      double *__restrict__ wo, double *__restrict__ part) {
   __shared__ double red[1][TPB / 64];
   double acc[1] = {0.0};
   for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (int64_t)MDB * TPB) {
     double sa = 0.0, sb = 0.0;
     for (int v = 0; v < nv; v++) {
-      const double x = V[v][i];
+      const double x = (double)V[v][i];
       sa = fma(ab[v], x, sa);
       sb = fma(ab[nv + v], x, sb);
     }
     const double uu = su * u[i];
-    u[i] = (uu - sa) * ia;
+    vo[i] = (VT)((uu - sa) * ia);
     const double r = w[i] - sb - gam * uu;
     wo[i] = r;
+    acc[0] = fma(r, r, acc[0]);
+  }
+  block_partials<1>(acc, red);
+  __syncthreads();
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+}
+
+// FP32-basis forms of the two passes: each thread takes element pairs (float2 of the basis,
+// double2 of u' and w), so a wave moves as many bytes per load as the FP64 forms; an odd last
+// element is added by thread 0 of block 0.  Same partial layout as k_mdot2 / k_dcgs_update.
+__global__ __launch_bounds__(TPB) void k_mdot2_f32(int64_t n, const float *const *__restrict__ V, int nv,
+                                                   const double *__restrict__ u, const double *__restrict__ w,
+                                                   double *__restrict__ part) {
+  constexpr int NR = 2 * MD2 + 3;
+  __shared__ double red[NR][TPB / 64];
+  const int64_t np = n >> 1;
+  const int64_t stride = (int64_t)MDB * TPB;
+  const int64_t i0 = (int64_t)blockIdx.x * TPB + threadIdx.x;
+  const bool tail = (n & 1) && i0 == 0;
+  const double2 *u2 = (const double2 *)u, *w2 = (const double2 *)w;
+  const int ngroups = nv > 0 ? (nv + MD2 - 1) / MD2 : 1;
+  for (int gi = 0; gi < ngroups; gi++) {
+    const int v0 = gi * MD2;
+    const int nm = max(0, min(MD2, nv - v0));
+    const bool first = gi == 0;
+    double s[NR];
+    const float2 *vp[MD2];
+#pragma unroll
+    for (int m = 0; m < NR; m++) s[m] = 0.0;
+#pragma unroll
+    for (int m = 0; m < MD2; m++) vp[m] = m < nm ? (const float2 *)V[v0 + m] : nullptr;  // read only for m < nm
+    int64_t i = i0;
+    for (; i + (MDU - 1) * stride < np; i += MDU * stride) {
+      double2 uv[MDU], wv[MDU];
+#pragma unroll
+      for (int q = 0; q < MDU; q++) {
+        uv[q] = u2[i + q * stride];
+        wv[q] = w2[i + q * stride];
+      }
+#pragma unroll
+      for (int m = 0; m < MD2; m++)
+        if (m < nm) {
+          float2 x[MDU];
+#pragma unroll
+          for (int q = 0; q < MDU; q++) x[q] = vp[m][i + q * stride];
+#pragma unroll
+          for (int q = 0; q < MDU; q++) {
+            s[2 * m] = fma((double)x[q].x, uv[q].x, s[2 * m]);
+            s[2 * m] = fma((double)x[q].y, uv[q].y, s[2 * m]);
+            s[2 * m + 1] = fma((double)x[q].x, wv[q].x, s[2 * m + 1]);
+            s[2 * m + 1] = fma((double)x[q].y, wv[q].y, s[2 * m + 1]);
+          }
+        }
+      if (first)
+#pragma unroll
+        for (int q = 0; q < MDU; q++) {
+          s[2 * MD2] = fma(uv[q].x, wv[q].x, s[2 * MD2]);
+          s[2 * MD2] = fma(uv[q].y, wv[q].y, s[2 * MD2]);
+          s[2 * MD2 + 1] = fma(uv[q].x, uv[q].x, s[2 * MD2 + 1]);
+          s[2 * MD2 + 1] = fma(uv[q].y, uv[q].y, s[2 * MD2 + 1]);
+          s[2 * MD2 + 2] = fma(wv[q].x, wv[q].x, s[2 * MD2 + 2]);
+          s[2 * MD2 + 2] = fma(wv[q].y, wv[q].y, s[2 * MD2 + 2]);
+        }
+    }
+    for (; i < np; i += stride) {
+      const double2 ui = u2[i], wi = w2[i];
+#pragma unroll
+      for (int m = 0; m < MD2; m++)
+        if (m < nm) {
+          const float2 x = vp[m][i];
+          s[2 * m] = fma((double)x.y, ui.y, fma((double)x.x, ui.x, s[2 * m]));
+          s[2 * m + 1] = fma((double)x.y, wi.y, fma((double)x.x, wi.x, s[2 * m + 1]));
+        }
+      if (first) {
+        s[2 * MD2] = fma(ui.y, wi.y, fma(ui.x, wi.x, s[2 * MD2]));
+        s[2 * MD2 + 1] = fma(ui.y, ui.y, fma(ui.x, ui.x, s[2 * MD2 + 1]));
+        s[2 * MD2 + 2] = fma(wi.y, wi.y, fma(wi.x, wi.x, s[2 * MD2 + 2]));
+      }
+    }
+    if (tail) {
+      const int64_t e = n - 1;
+#pragma unroll
+      for (int m = 0; m < MD2; m++)
+        if (m < nm) {
+          const double x = (double)V[v0 + m][e];
+          s[2 * m] = fma(x, u[e], s[2 * m]);
+          s[2 * m + 1] = fma(x, w[e], s[2 * m + 1]);
+        }
+      if (first) {
+        s[2 * MD2] = fma(u[e], w[e], s[2 * MD2]);
+        s[2 * MD2 + 1] = fma(u[e], u[e], s[2 * MD2 + 1]);
+        s[2 * MD2 + 2] = fma(w[e], w[e], s[2 * MD2 + 2]);
+      }
+    }
+    block_partials<NR>(s, red);
+    __syncthreads();
+    const int t = threadIdx.x;
+    int row = -1;
+    if (t < 2 * nm) row = 2 * v0 + t;
+    else if (first && t >= 2 * MD2 && t < NR) row = 2 * nv + (t - 2 * MD2);
+    if (row >= 0) part[(int64_t)row * MDB + blockIdx.x] = red[t][0] + red[t][1] + red[t][2] + red[t][3];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(TPB) void k_dcgs_update_f32(int64_t n, const float *const *__restrict__ V, int nv,
+                                                         const double *__restrict__ ab, double su, double ia,
+                                                         double gam, const double *__restrict__ u,
+                                                         float *__restrict__ vo, const double *__restrict__ w,
+                                                         double *__restrict__ wo, double *__restrict__ part) {
+  __shared__ double red[1][TPB / 64];
+  double acc[1] = {0.0};
+  const int64_t np = n >> 1;
+  const double2 *u2 = (const double2 *)u, *w2 = (const double2 *)w;
+  for (int64_t i = (int64_t)blockIdx.x * TPB + threadIdx.x; i < np; i += (int64_t)MDB * TPB) {
+    double sa0 = 0.0, sa1 = 0.0, sb0 = 0.0, sb1 = 0.0;
+    for (int v = 0; v < nv; v++) {
+      const float2 x = ((const float2 *)V[v])[i];
+      sa0 = fma(ab[v], (double)x.x, sa0);
+      sa1 = fma(ab[v], (double)x.y, sa1);
+      sb0 = fma(ab[nv + v], (double)x.x, sb0);
+      sb1 = fma(ab[nv + v], (double)x.y, sb1);
+    }
+    const double2 ui = u2[i], wi = w2[i];
+    const double uu0 = su * ui.x, uu1 = su * ui.y;
+    ((float2 *)vo)[i] = make_float2((float)((uu0 - sa0) * ia), (float)((uu1 - sa1) * ia));
+    const double r0 = wi.x - sb0 - gam * uu0, r1 = wi.y - sb1 - gam * uu1;
+    ((double2 *)wo)[i] = make_double2(r0, r1);
+    acc[0] = fma(r1, r1, fma(r0, r0, acc[0]));
+  }
+  if ((n & 1) && blockIdx.x == 0 && threadIdx.x == 0) {
+    const int64_t e = n - 1;
+    double sa = 0.0, sb = 0.0;
+    for (int v = 0; v < nv; v++) {
+      const double x = (double)V[v][e];
+      sa = fma(ab[v], x, sa);
+      sb = fma(ab[nv + v], x, sb);
+    }
+    const double uu = su * u[e];
+    vo[e] = (float)((uu - sa) * ia);
+    const double r = w[e] - sb - gam * uu;
+    wo[e] = r;
     acc[0] = fma(r, r, acc[0]);
   }
   block_partials<1>(acc, red);
@@ -430,6 +576,11 @@ __global__ void k_mcombine(int64_t n, const double *const *__restrict__ V, int n
     for (int v = 0; v < nv; v++) s += h[v] * V[v][i];
     x[i] = s;
   }
+}
+
+__global__ void k_to_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
 }
 
 __global__ void k_scale_copy(int64_t n, const double *__restrict__ a, double s, double *__restrict__ b) {
@@ -662,10 +813,13 @@ struct ptq2_mg {
   VcIO *io = nullptr;                   // input/output record of the V-cycle (level 0, FP64 vectors)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  // Krylov workspace
+  // Krylov workspace.  FP64 basis: Vk[j] holds candidate j, then basis vector j.  FP32 basis
+  // (basis32): candidates alternate between Vk[0] and Vk[1], finished vectors go to V32[j].
+  bool basis32 = false;
   std::vector<double *> Vk, Zk;
+  std::vector<float *> V32;
   double *w = nullptr, *part = nullptr, *psum = nullptr, *hdev = nullptr;
-  const double **Vptr = nullptr;  // device array of pointers (Vk)
+  const double **Vptr = nullptr;  // device table of the finished basis vectors (Vk or V32)
   const double **Zptr = nullptr;
   std::vector<double> hist;
   std::string last;
@@ -844,7 +998,11 @@ int vcycle_body(ptq2_mg *mg) {
 int mdot(ptq2_mg *mg, const double *const *Vdev, int nv, const double *w, int64_t n, bool self, double *out) {
   const int np = nv + (self ? 1 : 0);
   if (np == 0) return PTQ2_OK;
-  hipLaunchKernelGGL(k_mdot, dim3(MDB), dim3(TPB), 0, mg->stream, n, Vdev, nv, w, self ? 1 : 0, mg->part);
+  if (mg->basis32 && nv > 0)
+    hipLaunchKernelGGL(k_mdot<float>, dim3(MDB), dim3(TPB), 0, mg->stream, n, (const float *const *)Vdev, nv, w,
+                       self ? 1 : 0, mg->part);
+  else
+    hipLaunchKernelGGL(k_mdot<double>, dim3(MDB), dim3(TPB), 0, mg->stream, n, Vdev, nv, w, self ? 1 : 0, mg->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(np), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
   MGCHK(hipMemcpyAsync(out, mg->psum, sizeof(double) * np, hipMemcpyDeviceToHost, mg->stream));
   MGCHK(hipStreamSynchronize(mg->stream));
@@ -855,18 +1013,27 @@ int mdot(ptq2_mg *mg, const double *const *Vdev, int nv, const double *w, int64_
 // out[2nv..2nv+2] = u'.w, u'.u', w.w
 int mdot2(ptq2_mg *mg, int nv, const double *u, const double *w, int64_t n, double *out) {
   const int np = 2 * nv + 3;
-  hipLaunchKernelGGL(k_mdot2, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv, u, w, mg->part);
+  if (mg->basis32)
+    hipLaunchKernelGGL(k_mdot2_f32, dim3(MDB), dim3(TPB), 0, mg->stream, n, (const float *const *)mg->Vptr, nv, u, w,
+                       mg->part);
+  else
+    hipLaunchKernelGGL(k_mdot2<double>, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv, u, w, mg->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(np), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
   MGCHK(hipMemcpyAsync(out, mg->psum, sizeof(double) * np, hipMemcpyDeviceToHost, mg->stream));
   MGCHK(hipStreamSynchronize(mg->stream));
   return PTQ2_OK;
 }
 
-// pass 2 (coefficients [a, b] already in hdev); returns ||wo||
-int dcgs_update(ptq2_mg *mg, int nv, double su, double ia, double gam, double *u, double *wo, int64_t n,
-                double *nrm) {
-  hipLaunchKernelGGL(k_dcgs_update, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv, (const double *)mg->hdev,
-                     su, ia, gam, u, (const double *)mg->w, wo, mg->part);
+// pass 2 (coefficients [a, b] already in hdev): the finished vector into vo (the basis
+// precision), the next candidate into wo; returns ||wo||
+int dcgs_update(ptq2_mg *mg, int nv, double su, double ia, double gam, const double *u, void *vo, double *wo,
+                int64_t n, double *nrm) {
+  if (mg->basis32)
+    hipLaunchKernelGGL(k_dcgs_update_f32, dim3(MDB), dim3(TPB), 0, mg->stream, n, (const float *const *)mg->Vptr, nv,
+                       (const double *)mg->hdev, su, ia, gam, u, (float *)vo, (const double *)mg->w, wo, mg->part);
+  else
+    hipLaunchKernelGGL(k_dcgs_update<double>, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv,
+                       (const double *)mg->hdev, su, ia, gam, u, (double *)vo, (const double *)mg->w, wo, mg->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
   double d;
   MGCHK(hipMemcpyAsync(&d, mg->psum, sizeof(double), hipMemcpyDeviceToHost, mg->stream));
@@ -884,8 +1051,12 @@ int norm2(ptq2_mg *mg, const double *x, int64_t n, double *out) {
 
 // t -= sum_{v<nv} hdev[v] V[v] and the 2-norm of the result, in one pass
 int maxpy_norm(ptq2_mg *mg, int nv, double *t, int64_t n, double *nrm) {
-  hipLaunchKernelGGL(k_maxpy_norm, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv, (const double *)mg->hdev,
-                     t, mg->part);
+  if (mg->basis32)
+    hipLaunchKernelGGL(k_maxpy_norm<float>, dim3(MDB), dim3(TPB), 0, mg->stream, n, (const float *const *)mg->Vptr, nv,
+                       (const double *)mg->hdev, t, mg->part);
+  else
+    hipLaunchKernelGGL(k_maxpy_norm<double>, dim3(MDB), dim3(TPB), 0, mg->stream, n, mg->Vptr, nv,
+                       (const double *)mg->hdev, t, mg->part);
   hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(TPB), 0, mg->stream, (const double *)mg->part, mg->psum);
   double d;
   MGCHK(hipMemcpyAsync(&d, mg->psum, sizeof(double), hipMemcpyDeviceToHost, mg->stream));
@@ -1307,6 +1478,7 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->coarse_max_dofs = 20000;
   o->galerkin_coarse = 1;
   o->coarse_eta_mean = PTQ2_ETA_MEAN_ARITHMETIC;
+  o->fp32_basis = 0;
   o->rigid_deflation = 0;
   o->rigid_min_contrast = 100.0;
   o->rigid_max_inclusions = RG_MAXINC;
@@ -1326,6 +1498,7 @@ int ptq2_mg_destroy(ptq2_mg **pmg) {
     if (mg->owned[l]) ptq2_destroy(&mg->ctx[l]);
   if (!mg->ctx.empty() && mg->ctx[0]) ptq2_set_stream(mg->ctx[0], mg->fine_stream0);
   for (double *p : mg->Vk) (void)hipFree(p);
+  for (float *p : mg->V32) (void)hipFree(p);
   for (double *p : mg->Zk) (void)hipFree(p);
   for (void *p : {(void *)mg->ainv, (void *)mg->io, (void *)mg->w, (void *)mg->part, (void *)mg->psum,
                   (void *)mg->hdev, (void *)mg->Vptr, (void *)mg->Zptr, (void *)mg->rg_nodes, (void *)mg->rg_cchunk,
@@ -1363,6 +1536,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   mg->opt = o;
   mg->nl = o.nlevels;
   mg->f32 = o.fp32_smoother != 0;
+  mg->basis32 = o.fp32_basis != 0;
   // set-up and graph capture run on the fine context's own (capturable, non-null) stream
   mg->fine_stream0 = (void *)fine->stream;
   mg->stream = fine->own_stream;
@@ -1634,21 +1808,32 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
   const int64_t n = mg->n[0];
   ptq2_ctx *fc = mg->ctx[0];
   // workspace
-  if ((int)mg->Vk.size() < restart + 1) {
-    while ((int)mg->Vk.size() < restart + 1) {
-      double *p = nullptr;
-      MGRC(mg_alloc(mg, (void **)&p, sizeof(double) * n));
-      mg->Vk.push_back(p);
-    }
-    while ((int)mg->Zk.size() < restart) {
-      double *p = nullptr;
-      MGRC(mg_alloc(mg, (void **)&p, sizeof(double) * n));
-      mg->Zk.push_back(p);
-    }
+  const bool b32 = mg->basis32;
+  while ((int)mg->Vk.size() < (b32 ? 2 : restart + 1)) {
+    double *p = nullptr;
+    MGRC(mg_alloc(mg, (void **)&p, sizeof(double) * n));
+    mg->Vk.push_back(p);
+  }
+  while (b32 && (int)mg->V32.size() < restart) {
+    float *p = nullptr;
+    MGRC(mg_alloc(mg, (void **)&p, sizeof(float) * n));
+    mg->V32.push_back(p);
+  }
+  while ((int)mg->Zk.size() < restart) {
+    double *p = nullptr;
+    MGRC(mg_alloc(mg, (void **)&p, sizeof(double) * n));
+    mg->Zk.push_back(p);
   }
   if (!mg->w) MGRC(mg_alloc(mg, (void **)&mg->w, sizeof(double) * n));
-  std::vector<const double *> vp(mg->Vk.begin(), mg->Vk.end()), zp(mg->Zk.begin(), mg->Zk.end());
-  MGCHK(hipMemcpy((void *)mg->Vptr, vp.data(), sizeof(double *) * vp.size(), hipMemcpyHostToDevice));
+  // candidate j (FP64) and finished basis vector j (basis precision); the pointer table lists
+  // the finished vectors
+  auto cand = [&](int j) { return b32 ? mg->Vk[j & 1] : mg->Vk[j]; };
+  auto fin = [&](int j) -> void * { return b32 ? (void *)mg->V32[j] : (void *)mg->Vk[j]; };
+  std::vector<const void *> vp;
+  if (b32) vp.assign(mg->V32.begin(), mg->V32.end());
+  else vp.assign(mg->Vk.begin(), mg->Vk.end());
+  std::vector<const double *> zp(mg->Zk.begin(), mg->Zk.end());
+  MGCHK(hipMemcpy((void *)mg->Vptr, vp.data(), sizeof(void *) * vp.size(), hipMemcpyHostToDevice));
   MGCHK(hipMemcpy((void *)mg->Zptr, zp.data(), sizeof(double *) * zp.size(), hipMemcpyHostToDevice));
   ptq2_set_stream(fc, mg->stream);
   mg->hist.clear();
@@ -1697,9 +1882,9 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
   while (it < max_it) {
     // r = b - A x, the unnormalised first candidate
     MGRC(ptq2_a11_apply(fc, x, mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
-    hipLaunchKernelGGL(k_sub, dim3(gb), dim3(TPB), 0, mg->stream, n, b, (const double *)mg->w, mg->Vk[0]);
+    hipLaunchKernelGGL(k_sub, dim3(gb), dim3(TPB), 0, mg->stream, n, b, (const double *)mg->w, cand(0));
     double beta;
-    MGRC(norm2(mg, mg->Vk[0], n, &beta));
+    MGRC(norm2(mg, cand(0), n, &beta));
     if (beta <= target) {
       rn = beta;
       break;
@@ -1711,10 +1896,10 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
     double bprev = 0.0;      // H(j, j-1) before the reorthogonalisation of u_j
     int k = 0;               // finished columns of this cycle
     for (int j = 0; j < restart && it < max_it; j++) {
-      MGRC(vcycle_scaled(mg, mg->Vk[j], su, mg->Zk[j]));
+      MGRC(vcycle_scaled(mg, cand(j), su, mg->Zk[j]));
       MGRC(ptq2_a11_apply(fc, mg->Zk[j], mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
       it++;
-      MGRC(mdot2(mg, j, mg->Vk[j], mg->w, n, dots.data()));
+      MGRC(mdot2(mg, j, cand(j), mg->w, n, dots.data()));
       double aa = 0.0, abd = 0.0;
       for (int v = 0; v < j; v++) {
         av[v] = su * dots[2 * v];
@@ -1730,13 +1915,14 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
       } else {
         // u_j lost more than half its norm to V: finish it explicitly and project w by
         // plain CGS2 against the finished basis
-        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->Vk[j], su,
-                           mg->Vk[j]);
+        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)cand(j), su, cand(j));
         if (j > 0) MGCHK(hipMemcpyAsync(mg->hdev, av.data(), sizeof(double) * j, hipMemcpyHostToDevice, mg->stream));
-        MGRC(maxpy_norm(mg, j, mg->Vk[j], n, &alpha));
+        MGRC(maxpy_norm(mg, j, cand(j), n, &alpha));
         if (!(alpha > 0.0)) return mgfail(mg, PTQ2_ERR_DEVICE, "FGMRES: Krylov candidate vanished");
-        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)mg->Vk[j], 1.0 / alpha,
-                           mg->Vk[j]);
+        hipLaunchKernelGGL(k_scale_copy, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)cand(j), 1.0 / alpha,
+                           cand(j));
+        if (b32)
+          hipLaunchKernelGGL(k_to_f32, dim3(gb), dim3(TPB), 0, mg->stream, n, (const double *)cand(j), mg->V32[j]);
         hjj = std::nan("");
       }
       // column j-1 is now final
@@ -1758,7 +1944,7 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
         }
         Hc(j, j) = hjj;
         if (j > 0) MGCHK(hipMemcpyAsync(mg->hdev, ab.data(), sizeof(double) * 2 * j, hipMemcpyHostToDevice, mg->stream));
-        MGRC(dcgs_update(mg, j, su, 1.0 / alpha, gam, mg->Vk[j], mg->Vk[j + 1], n, &bn));
+        MGRC(dcgs_update(mg, j, su, 1.0 / alpha, gam, cand(j), fin(j), cand(j + 1), n, &bn));
       } else {
         // CGS with the DGKS second pass against v_0..v_j
         std::vector<double> h(j + 2), h2(j + 2);
@@ -1773,7 +1959,7 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
           for (int i = 0; i <= j; i++) h[i] += h2[i];
         }
         for (int i = 0; i <= j; i++) Hc(i, j) = h[i];
-        MGCHK(hipMemcpyAsync(mg->Vk[j + 1], mg->w, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
+        MGCHK(hipMemcpyAsync(cand(j + 1), mg->w, sizeof(double) * n, hipMemcpyDeviceToDevice, mg->stream));
       }
       Hc(j + 1, j) = bn;
       bprev = bn;
@@ -1786,7 +1972,7 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
       su = 1.0 / bn;
       if (j == restart - 1 || it >= max_it) {
         // end of the cycle: finish column j with the reorthogonalisation of u_{j+1}
-        MGRC(mdot(mg, Vp, j + 1, mg->Vk[j + 1], n, true, dots.data()));
+        MGRC(mdot(mg, Vp, j + 1, cand(j + 1), n, true, dots.data()));
         double a2 = 0.0;
         for (int v = 0; v <= j; v++) {
           const double a = su * dots[v];

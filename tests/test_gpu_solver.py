@@ -119,6 +119,30 @@ def test_fgmres_solve_16():
     assert its[True] <= its[False] + max(3, its[False] // 10)
 
 
+@pytest.mark.parametrize("restart", [30, 7])
+def test_fgmres_fp32_basis(restart):
+    """FP32 storage of the finished Arnoldi vectors: the FP64 true residual still reaches rtol
+    1e-8 (checked with the restatement's operator), with at most a few more iterations than the
+    FP64 basis; restart 7 crosses many cycles (each recomputes the FP64 residual)."""
+    m = 16
+    P, ctx = make(m, 1e4, n_incl=6)
+    xs = M.uniform(22, P.nvel, -1, 1)
+    xs[P.mask > 0] = 0.0
+    b = O.matmult_a11(m, m, m, P.coords, P.eta, P.mask, xs)
+    its = {}
+    for b32 in (False, True):
+        say("case: FGMRES(%d) 16^3 contrast 1e4, FP32 smoother, fp32 basis=%s" % (restart, b32))
+        S = ViscousSolver(ctx, nlevels=4, smooth_its=4, fp32=True, fp32_basis=b32)
+        x, info = S.solve(torch.from_numpy(b).to(DEV), rtol=1e-8, max_it=600, restart=restart)
+        x = x.cpu().numpy()
+        say("  iterations %d, rel residual %.3e" % (info["iterations"], info["rel"]))
+        r = b - O.matmult_a11(m, m, m, P.coords, P.eta, P.mask, x)
+        assert np.linalg.norm(r) <= 1.05e-8 * np.linalg.norm(b)
+        its[b32] = info["iterations"]
+        S.destroy()
+    assert its[True] <= its[False] + max(3, its[False] // 10)
+
+
 def test_vcycle_large_coarse_level():
     """Two levels, 12^3 -> 6^3: the coarse dense inverse (6591 dofs, 103 sweep blocks, padded)
     against the restatement's LAPACK inverse."""
