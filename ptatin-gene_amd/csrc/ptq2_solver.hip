@@ -687,27 +687,25 @@ __global__ __launch_bounds__(TPB) void k_rg_moments(const T *__restrict__ x, con
   }
 }
 
-__device__ __forceinline__ double rg_msum(int t, const int *__restrict__ cchunk, const double *__restrict__ part) {
-  const int c = t / 6, k = t % 6;
-  double s = 0.0;
-  for (int q = cchunk[c]; q < cchunk[c + 1]; q++) s += part[6 * (int64_t)q + k];
-  return s;
-}
-
-// out[t] = moment t of the last k_rg_moments pass (set-up: a column of E)
-__global__ void k_rg_msum(int n6, const int *__restrict__ cchunk, const double *__restrict__ part,
-                          double *__restrict__ out) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < n6) out[t] = rg_msum(t, cchunk, part);
-}
-
-// y = E^-1 m (m: the chunk moments summed per inclusion in chunk order); 4 rows per block
-__global__ __launch_bounds__(256) void k_rg_coarse(int n6, const int *__restrict__ cchunk,
-                                                  const double *__restrict__ part, const double *__restrict__ einv,
-                                                  double *__restrict__ y) {
-  __shared__ double m[6 * RG_MAXINC];
-  for (int t = threadIdx.x; t < n6; t += 256) m[t] = rg_msum(t, cchunk, part);
+// m[6 c + k] = moment k of inclusion c: its chunk partials summed by one block per inclusion
+// (fixed strides and tree, so reproducible); set-up: m is a column of E
+__global__ __launch_bounds__(TPB) void k_rg_msum(const int *__restrict__ cchunk, const double *__restrict__ part,
+                                                double *__restrict__ m) {
+  __shared__ double red[6][TPB / 64];
+  const int c = blockIdx.x;
+  double s[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int q = cchunk[c] + (int)threadIdx.x; q < cchunk[c + 1]; q += TPB)
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[k] += part[6 * (int64_t)q + k];
+  block_partials<6>(s, red);
   __syncthreads();
+  if (threadIdx.x < 6) m[6 * c + threadIdx.x] = red[threadIdx.x][0] + red[threadIdx.x][1] + red[threadIdx.x][2] +
+                                                red[threadIdx.x][3];
+}
+
+// y = E^-1 m; 4 rows per block, one wave per row
+__global__ __launch_bounds__(256) void k_rg_coarse(int n6, const double *__restrict__ m,
+                                                  const double *__restrict__ einv, double *__restrict__ y) {
   const int row = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (row >= n6) return;  // whole waves leave together
   double s = 0.0;
@@ -831,7 +829,7 @@ struct ptq2_mg {
   int *rg_nodes = nullptr, *rg_cchunk = nullptr, *rg_dnode = nullptr, *rg_dptr = nullptr, *rg_ecomp = nullptr;
   RgChunk *rg_chunk = nullptr;
   double4 *rg_cent = nullptr;
-  double *rg_part = nullptr, *rg_y = nullptr, *rg_einv = nullptr, *rg_aw = nullptr;
+  double *rg_part = nullptr, *rg_m = nullptr, *rg_y = nullptr, *rg_einv = nullptr, *rg_aw = nullptr;
 };
 
 namespace {
@@ -955,8 +953,10 @@ void rigid_pre(ptq2_mg *mg, T *b) {
   hipLaunchKernelGGL(k_rg_moments<T>, dim3(mg->rg_nchunk), dim3(TPB), 0, mg->stream, (const T *)b,
                      (const uint8_t *)f->mask, (const double *)f->coords, (const int *)mg->rg_nodes,
                      (const RgChunk *)mg->rg_chunk, (const double4 *)mg->rg_cent, mg->rg_part);
+  hipLaunchKernelGGL(k_rg_msum, dim3(mg->rg_nc), dim3(TPB), 0, mg->stream, (const int *)mg->rg_cchunk,
+                     (const double *)mg->rg_part, mg->rg_m);
   hipLaunchKernelGGL(k_rg_coarse, dim3((mg->rg_n6 + 3) / 4), dim3(256), 0, mg->stream, mg->rg_n6,
-                     (const int *)mg->rg_cchunk, (const double *)mg->rg_part, (const double *)mg->rg_einv, mg->rg_y);
+                     (const double *)mg->rg_m, (const double *)mg->rg_einv, mg->rg_y);
   hipLaunchKernelGGL(k_rg_subtract<T>, dim3(nblk(mg->rg_nd)), dim3(TPB), 0, mg->stream, mg->rg_nd,
                      (const int *)mg->rg_dnode, (const int *)mg->rg_dptr, (const int *)mg->rg_ecomp,
                      (const double *)mg->rg_aw, (const double *)mg->rg_y, b);
@@ -1378,6 +1378,7 @@ int rigid_setup(ptq2_mg *mg) {
   MGRC(up((void **)&mg->rg_ecomp, ecomp.data(), sizeof(int) * ecomp.size()));
   MGRC(mg_alloc(mg, (void **)&mg->rg_part, sizeof(double) * 6 * chunks.size()));
   MGRC(mg_alloc(mg, (void **)&mg->rg_y, sizeof(double) * n6));
+  MGRC(mg_alloc(mg, (void **)&mg->rg_m, sizeof(double) * n6));
   MGRC(mg_alloc(mg, (void **)&mg->rg_einv, sizeof(double) * n6 * n6));
   MGRC(mg_alloc(mg, (void **)&mg->rg_aw, sizeof(double) * 18 * pr.size()));
   // columns of E and of A W
@@ -1393,7 +1394,7 @@ int rigid_setup(ptq2_mg *mg) {
     hipLaunchKernelGGL(k_rg_moments<double>, dim3(mg->rg_nchunk), dim3(TPB), 0, mg->stream, (const double *)v,
                        (const uint8_t *)f->mask, (const double *)f->coords, (const int *)mg->rg_nodes,
                        (const RgChunk *)mg->rg_chunk, (const double4 *)mg->rg_cent, mg->rg_part);
-    hipLaunchKernelGGL(k_rg_msum, dim3((n6 + 255) / 256), dim3(256), 0, mg->stream, n6, (const int *)mg->rg_cchunk,
+    hipLaunchKernelGGL(k_rg_msum, dim3(nc), dim3(TPB), 0, mg->stream, (const int *)mg->rg_cchunk,
                        (const double *)mg->rg_part, mg->rg_einv + (int64_t)j * n6);
     hipLaunchKernelGGL(k_rg_gather, dim3(nblk(mg->rg_nd)), dim3(TPB), 0, mg->stream, mg->rg_nd,
                        (const int *)mg->rg_dnode, (const int *)mg->rg_dptr, (const int *)mg->rg_ecomp, j / 6, j % 6,
@@ -1503,7 +1504,7 @@ int ptq2_mg_destroy(ptq2_mg **pmg) {
   for (void *p : {(void *)mg->ainv, (void *)mg->io, (void *)mg->w, (void *)mg->part, (void *)mg->psum,
                   (void *)mg->hdev, (void *)mg->Vptr, (void *)mg->Zptr, (void *)mg->rg_nodes, (void *)mg->rg_cchunk,
                   (void *)mg->rg_dnode, (void *)mg->rg_dptr, (void *)mg->rg_ecomp, (void *)mg->rg_chunk,
-                  (void *)mg->rg_cent, (void *)mg->rg_part, (void *)mg->rg_y, (void *)mg->rg_einv, (void *)mg->rg_aw})
+                  (void *)mg->rg_cent, (void *)mg->rg_part, (void *)mg->rg_m, (void *)mg->rg_y, (void *)mg->rg_einv, (void *)mg->rg_aw})
     if (p) (void)hipFree(p);
   delete mg;
   *pmg = nullptr;
