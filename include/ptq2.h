@@ -100,6 +100,12 @@ int ptq2_a11_diag(ptq2_ctx *ctx, double *d, unsigned flags);
    (src/stokes_operators_avx.c:682-829); PTQ2_MATMULT is MatMult_MFStokes_A12
    (src/stokes_operators.c:1301-1370): Dirichlet rows of y set to 0. */
 int ptq2_a12_apply(ptq2_ctx *ctx, const double *p, double *yu, unsigned flags);
+/* body-force load: yu[3n+d] += sum_e sum_q w_q detJ_q N_n(x_q) fu[(27e+q)*3+d], the Be term of
+   FormFunctionLocal_U (src/stokes_form_function.c:219-224) with fu the Fu[3] of
+   QPntVolCoefStokes (include/QPntVolCoefStokes_def.h:41-46); 81E doubles in.  The
+   momentum residual of the reference is A11 u + A12 p - this vector.  PTQ2_MATMULT: Dirichlet
+   rows of yu set to 0 (the right-hand side of the constrained solve). */
+int ptq2_body_force(ptq2_ctx *ctx, const double *fu, double *yu, unsigned flags);
 /* divergence A21 (velocity in, pressure 4E out): MFStokesWrapper_A21
    (src/stokes_operators_mf.c:862-937); PTQ2_MATMULT is MatMult_MFStokes_A21
    (src/stokes_operators.c:1444-1509): Dirichlet columns of u zeroed. */

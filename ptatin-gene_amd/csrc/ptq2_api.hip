@@ -468,6 +468,36 @@ int ptq2_a12_apply(ptq2_ctx *c, const double *p, double *yu, unsigned flags) {
   return PTQ2_OK;
 }
 
+int ptq2_body_force(ptq2_ctx *c, const double *fu, double *y, unsigned flags) {
+  int rc = check_ready(c, false);
+  if (rc) return rc;
+  if (!fu || !y) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
+  if (flags & ~(PTQ2_DEVICE | PTQ2_MATMULT | PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
+  if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
+  if ((rc = set_device(c))) return rc;
+  const bool dev = flags & PTQ2_DEVICE, mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
+  const double *df = fu;
+  double *dy = y;
+  if (!dev) {
+    if ((rc = clear_err(c))) return rc;
+    // the pressure staging buffer, grown to the 81 values per element of fu
+    if ((rc = stage_in<double>(c, fu, (void **)&c->sp, &c->sp_bytes, 81 * c->nel, true))) return rc;
+    if ((rc = ensure(c, &c->sy, &c->sy_bytes, sizeof(double) * (size_t)c->nvel))) return rc;
+    if (add) HIPCHK(c, hipMemcpyAsync(c->sy, y, sizeof(double) * (size_t)c->nvel, hipMemcpyHostToDevice, c->stream));
+    df = c->sp;
+    dy = (double *)c->sy;
+  }
+  HIPCHK(c, ptq2::launch_body_force(c->box, c->coords, df, dy, add, c->stream));
+  if (mm) HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, (const double *)nullptr, dy, 1, c->stream));
+  if (!dev) {
+    HIPCHK(c, hipMemcpyAsync(y, dy, sizeof(double) * (size_t)c->nvel, hipMemcpyDeviceToHost, c->stream));
+    int bits = 0;
+    if ((rc = read_err(c, &bits))) return rc;
+    if (bits & ptq2::ERRBIT_NONFINITE) return fail(c, PTQ2_ERR_NONFINITE, "non-finite value in the body force");
+  }
+  return PTQ2_OK;
+}
+
 int ptq2_a21_apply(ptq2_ctx *c, const double *u, double *yp, unsigned flags) {
   int rc = check_ready(c, false);
   if (rc) return rc;

@@ -32,6 +32,9 @@ hipError_t launch_a21(const Box &b, const double *coords, const double *u, const
                       hipStream_t s);
 hipError_t launch_a12(const Box &b, const double *coords, const double *p, double *y, bool add, hipStream_t s);
 hipError_t launch_diag(const Box &b, const double *coords, const double *eta_bm, double *d, bool add, hipStream_t s);
+// y (+)= the body-force load of fu (3 per quadrature point, element-major)
+hipError_t launch_body_force(const Box &b, const double *coords, const double *fu, double *y, bool add,
+                             hipStream_t s);
 template <typename T>
 hipError_t launch_bc_rows(int64_t n, const int64_t *idx, const T *x, T *y, int mode, hipStream_t s);
 template <typename T>

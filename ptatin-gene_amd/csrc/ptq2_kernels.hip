@@ -814,6 +814,87 @@ __global__ __launch_bounds__(BEL) void k_a12(Box b, int color, const double *__r
   store_tile(b, br, t, Xs, y, add_mode != 0, lane);
 }
 
+// ---------------------------------------------------------------- body force
+// y[3n+d] += sum_e sum_q w_q det J_q N_n(x_q) fu[e][q][d]: the Be term of FormFunctionLocal_U
+// (src/stokes_form_function.c:219-224), the momentum load of buoyancy problems.  Per element the
+// 27 point values w det fu, then the transposed Q2 interpolation one axis at a time.
+
+template <int qa, int qb, int qc>
+__device__ __forceinline__ void bf_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                      const double *__restrict__ fe, double F[3][27]) {
+  double J[3][3], A[3][3];
+  qp_grad<qa>(xr0, xr1, xr2, J);
+  const double det = adjugate(J, A);
+  constexpr int q = qa + 3 * qb + 9 * qc;
+  constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
+  const double s = w * det;
+#pragma unroll
+  for (int d = 0; d < 3; d++) F[d][q] = s * fe[3 * q + d];
+}
+
+template <int qb, int qc>
+__device__ __forceinline__ void bf_row(const double *Xs, int base, const double *__restrict__ fe, double F[3][27]) {
+  double xr0[3][3], xr1[3][3], xr2[3][3];
+  row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
+  bf_qp<0, qb, qc>(xr0, xr1, xr2, fe, F);
+  bf_qp<1, qb, qc>(xr0, xr1, xr2, fe, F);
+  bf_qp<2, qb, qc>(xr0, xr1, xr2, fe, F);
+}
+
+__global__ __launch_bounds__(BEL) void k_body_force(Box b, int color, const double *__restrict__ coords,
+                                                    const double *__restrict__ fu, double *__restrict__ y,
+                                                    int add_mode) {
+  __shared__ double Xs[3 * TCS];
+  const int lane = threadIdx.x;
+  const BrickId br = brick_of(b, color, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  load_tile(b, t, br, coords, (const uint8_t *)nullptr, Xs, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  double Y[3][3][3][3];
+  if (active) {
+    const double *fe = fu + 81 * lane_element(b, t, lane);
+    double F[3][27];
+    bf_row<0, 0>(Xs, base, fe, F); bf_row<1, 0>(Xs, base, fe, F); bf_row<2, 0>(Xs, base, fe, F);
+    bf_row<0, 1>(Xs, base, fe, F); bf_row<1, 1>(Xs, base, fe, F); bf_row<2, 1>(Xs, base, fe, F);
+    bf_row<0, 2>(Xs, base, fe, F); bf_row<1, 2>(Xs, base, fe, F); bf_row<2, 2>(Xs, base, fe, F);
+    // B^T along x, then y, then z: Y[d][k][j][i] = sum B(qa,i) B(qb,j) B(qc,k) F[d][qa+3qb+9qc]
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+      double T1[3][3][3], T2[3][3][3];  // T1[qc][qb][i], T2[qc][j][i]
+#pragma unroll
+      for (int qc = 0; qc < 3; qc++)
+#pragma unroll
+        for (int qb = 0; qb < 3; qb++)
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+            T1[qc][qb][i] = gB(0, i) * F[d][9 * qc + 3 * qb] + gB(1, i) * F[d][9 * qc + 3 * qb + 1] +
+                            gB(2, i) * F[d][9 * qc + 3 * qb + 2];
+#pragma unroll
+      for (int qc = 0; qc < 3; qc++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+            T2[qc][j][i] = gB(0, j) * T1[qc][0][i] + gB(1, j) * T1[qc][1][i] + gB(2, j) * T1[qc][2][i];
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++)
+#pragma unroll
+          for (int i = 0; i < 3; i++)
+            Y[d][k][j][i] = gB(0, k) * T2[0][j][i] + gB(1, k) * T2[1][j][i] + gB(2, k) * T2[2][j][i];
+    }
+  }
+  __syncthreads();
+  zero_tile(Xs, lane);
+  __syncthreads();
+  if (active) emit_element(Xs, base, Y);
+  __syncthreads();
+  store_tile(b, br, t, Xs, y, add_mode != 0, lane);
+}
+
 // ---------------------------------------------------------------- diagonal
 // diag(n,x) += sum_q w det eta (2 Nx^2 + Ny^2 + Nz^2) (cyclic for y, z), the form of
 // MatMultMF_Stokes_MixedFEM3d_diagB11 (src/stokes_q2p1_mf_operators_diag_def.c:120-205).
@@ -1114,6 +1195,16 @@ hipError_t launch_a12(const Box &b, const double *coords, const double *p, doubl
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
     hipLaunchKernelGGL(k_a12, dim3(nb), dim3(BEL), 0, s, b, c, coords, p, y, add ? 1 : 0);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_body_force(const Box &b, const double *coords, const double *fu, double *y, bool add,
+                             hipStream_t s) {
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL(k_body_force, dim3(nb), dim3(BEL), 0, s, b, c, coords, fu, y, add ? 1 : 0);
   }
   return hipGetLastError();
 }

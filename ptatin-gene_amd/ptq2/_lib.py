@@ -44,6 +44,7 @@ SIGNATURES = {
     "ptq2_a11_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_a11_diag": (_c_int, [_vp, _vp, _c_uint]),
     "ptq2_a12_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
+    "ptq2_body_force": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_a21_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_stokes_apply": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_uint]),
     "ptq2_sizes": (_c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),

@@ -134,6 +134,13 @@ class Context:
         dt = torch.float64 if _is_dev(p) else np.float64
         return self._call1(L.ptq2_a12_apply, "ptq2_a12_apply", p, y, flags, np.float64, dt, self.nvel)
 
+    def body_force(self, fu, y=None, flags=0):
+        """Body-force load of fu (E x 27 x 3 quadrature values, element-major): the Be term of
+        FormFunctionLocal_U (src/stokes_form_function.c:219-224)."""
+        L = _lib.lib()
+        dt = torch.float64 if _is_dev(fu) else np.float64
+        return self._call1(L.ptq2_body_force, "ptq2_body_force", fu, y, flags, np.float64, dt, self.nvel)
+
     def a21(self, u, y=None, flags=0):
         L = _lib.lib()
         dt = torch.float64 if _is_dev(u) else np.float64

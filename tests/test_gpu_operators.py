@@ -337,3 +337,34 @@ def test_full_size_properties_128():
     A.ctx.set_viscosity(np.ones(27 * m ** 3), 99)
     e = float(lin @ A.ctx.a11(lin))
     assert abs(e - 2.0) <= 1e-10
+
+
+@pytest.mark.parametrize("dims", [(8, 9, 11), (16, 16, 16), (5, 7, 14)])
+def test_body_force_matches_host_restatement(dims):
+    """ptq2_body_force against the host restatement ptq2.mesh.body_force (pinned by pointwise
+    assembly and by the integral identity in tests/test_host.py) on jittered meshes with partial
+    bricks: FP64 1e-12; host and device pointers bitwise equal; ADD accumulates; MATMULT zeroes
+    the Dirichlet rows; a non-finite input is reported."""
+    mx, my, mz = dims
+    say("case: body force %dx%dx%d" % dims)
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 3)
+    mask = M.bc_mask(mx, my, mz, "freeslip")
+    fu = M.uniform(31, 81 * mx * my * mz, -1, 1).reshape(-1, 27, 3)
+    ref = M.body_force(mx, my, mz, c, fu)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    y = ctx.body_force(fu)
+    assert rel(y, ref) <= 1e-12
+    yd = ctx.body_force(torch.from_numpy(fu).to(DEV))
+    torch.cuda.synchronize()
+    assert np.array_equal(yd.cpu().numpy(), y)
+    y0 = M.uniform(32, ref.size, -1, 1)
+    ya = ctx.body_force(fu, y0.copy(), _lib.ADD)
+    assert rel(ya, y0 + ref) <= 1e-12
+    ym = ctx.body_force(fu, None, _lib.MATMULT)
+    assert np.all(ym[mask > 0] == 0.0) and np.array_equal(ym[mask == 0], y[mask == 0])
+    bad = fu.copy()
+    bad[3, 4, 1] = np.nan
+    with pytest.raises(ptq2.PTQ2Error):
+        ctx.body_force(bad)
+    ctx.destroy()
