@@ -125,6 +125,32 @@ def block_comparison(m, steps, warmup, stream):
     return out
 
 
+def body_force_timing(m, steps, warmup, stream):
+    """The body-force load (ptq2_body_force) on the 128^3 mesh from uniform random quadrature
+    values: time per call and its bandwidth over the algorithmic bytes (fu 81 x 8 B per element,
+    coordinates and y 3 x 8 B per node)."""
+    import torch
+
+    import ptq2
+    from ptq2 import mesh as M
+
+    c = M.jitter(m, m, m, M.q2_uniform(m, m, m), 0.1, 1.0 / (2 * m), 1)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(m, m, m, c)
+    del c
+    g = torch.Generator(device="cuda:0")
+    g.manual_seed(11)
+    fu = torch.rand(81 * ctx.nel, dtype=torch.float64, device="cuda:0", generator=g)
+    y = torch.empty(ctx.nvel, dtype=torch.float64, device="cuda:0")
+    ms = time_applies(lambda: ctx.body_force(fu, y), steps, warmup, stream)
+    nbytes = 8 * (81 * ctx.nel + 2 * ctx.nvel)
+    out = {"workload": "body-force load, %d^3 Q2 elements, jittered mesh, random quadrature values" % m,
+           "ms": round(ms, 4), "algorithmic_gbs": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+    del fu, y
+    ctx.destroy()
+    return out
+
+
 def cpu_baseline(seconds=12.0):
     """The oracle's restatement of the reference's default sum-factorized CPU algorithm
     (OpenMP over race-free element colours) on a bounded 40^3 sample."""
@@ -351,6 +377,11 @@ def main():
             extra["block"] = block_comparison(m, args.steps, args.warmup, stream)
         except Exception as exc:
             extra["block"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+    if args.block:
+        try:
+            extra["body_force"] = body_force_timing(m, args.steps, args.warmup, stream)
+        except Exception as exc:
+            extra["body_force"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     variants = {"off": [False], "on": [True], "both": [False, True]}
     for rigid in variants[args.rigid] if args.solve else []:
         key = "solve_rigid" if rigid and args.rigid == "both" else "solve"
