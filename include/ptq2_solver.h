@@ -57,7 +57,18 @@ typedef struct {
   int rigid_max_inclusions;  /* default 128 (1..128) */
 } ptq2_mg_opts;
 
-enum { PTQ2_ETA_MEAN_ARITHMETIC = 0, PTQ2_ETA_MEAN_GEOMETRIC = 1, PTQ2_ETA_MEAN_HARMONIC = 2, PTQ2_ETA_MEAN_MAX = 3 };
+/* PTQ2_ETA_MEAN_Q1PROJ: the reference's coefficient hierarchy
+   (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes_Hierarchy,
+   src/material_point_utils.c:1325-1524, projection type 0): a nodal field on the fine Q2 node
+   lattice (Q1 weights on the element corners, 1 elsewhere), restricted lattice by lattice with
+   the scaled trilinear restriction, Q1-interpolated to each coarse level's quadrature points */
+enum {
+  PTQ2_ETA_MEAN_ARITHMETIC = 0,
+  PTQ2_ETA_MEAN_GEOMETRIC = 1,
+  PTQ2_ETA_MEAN_HARMONIC = 2,
+  PTQ2_ETA_MEAN_MAX = 3,
+  PTQ2_ETA_MEAN_Q1PROJ = 4
+};
 
 /* fill opts with the defaults: Chebyshev(4), FP32 smoothing, (0.2, 1.1) transform,
    20 power iterations, graph replay, coarse dense limit 20000 dofs, Galerkin coarsest

@@ -61,6 +61,108 @@ def _eta_coarsen_max(mx, my, mz, eta):
     return out.reshape(-1)
 
 
+# ---------------------------------------------------------------------------------------
+# The reference's coefficient hierarchy (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_-
+# MPntPStokes_Hierarchy, src/material_point_utils.c:1325-1524, default projection type 0),
+# ptq2_mg_opts.coarse_eta_mean = PTQ2_ETA_Q1_PROJECTION:
+#  1. fine nodal field on the Q2 node lattice (_BuildQ1CoefficientProjection_QuadraturePoints_-
+#     MPntPStokes_FineGrid, :1196-1300): per element and quadrature point, the 8 corner nodes
+#     get Q1 weights N_i(xi_p), every other node of the element weight 1; node value =
+#     sum(weight * eta_p) / sum(weight) over all elements;
+#  2. each coarser lattice: the scalar trilinear restriction scaled by 1 / (R 1)
+#     (MatRestrict + DMCreateInterpolationScale, :1450-1462);
+#  3. coarse quadrature values: Q1 interpolation from the element's corner nodes, clipped to
+#     the range of its 27 nodal values (_..._InterpolateToQuadratePoints, :957-1070).
+# The fine level keeps its own quadrature values.
+# ---------------------------------------------------------------------------------------
+_G3 = np.array([-np.sqrt(0.6), 0.0, np.sqrt(0.6)])
+
+
+def _q1_at_gauss():
+    """N[p, c] = Q1 shape function of corner c = cx + 2cy + 4cz at Gauss point p = i + 3j + 9k"""
+    N = np.zeros((27, 8))
+    for k in range(3):
+        for j in range(3):
+            for i in range(3):
+                xi = (_G3[i], _G3[j], _G3[k])
+                for c in range(8):
+                    s = [1 if (c >> a) & 1 else -1 for a in range(3)]
+                    N[i + 3 * j + 9 * k, c] = 0.125 * (1 + s[0] * xi[0]) * (1 + s[1] * xi[1]) * (1 + s[2] * xi[2])
+    return N
+
+
+def q1_nodal_projection(mx, my, mz, eta):
+    e = np.asarray(eta, dtype=np.float64).reshape(mz, my, mx, 27)
+    N = _q1_at_gauss()
+    A = np.zeros((2 * mz + 1, 2 * my + 1, 2 * mx + 1))
+    B = np.zeros_like(A)
+    corner = {0: 0, 2: 1, 6: 2, 8: 3, 18: 4, 20: 5, 24: 6, 26: 7}
+    for n in range(27):
+        i, j, k = n % 3, (n // 3) % 3, n // 9
+        if n in corner:
+            w = N[:, corner[n]]
+            a = e @ w
+            b = w.sum()
+        else:
+            a = e.sum(axis=3)
+            b = 27.0
+        A[k:k + 2 * mz:2, j:j + 2 * my:2, i:i + 2 * mx:2] += a
+        B[k:k + 2 * mz:2, j:j + 2 * my:2, i:i + 2 * mx:2] += b
+    return A / B
+
+
+def _restrict_scaled_1d(a, axis):
+    a = np.moveaxis(a, axis, 0)
+    nc = (a.shape[0] - 1) // 2 + 1
+    num = np.zeros((nc,) + a.shape[1:])
+    den = np.zeros(nc)
+    for f in range(a.shape[0]):
+        if f % 2 == 0:
+            num[f // 2] += a[f]
+            den[f // 2] += 1.0
+        else:
+            for c in ((f - 1) // 2, (f + 1) // 2):
+                num[c] += 0.5 * a[f]
+                den[c] += 0.5
+    return np.moveaxis(num / den.reshape((-1,) + (1,) * (a.ndim - 1)), 0, axis)
+
+
+def q1_restrict_nodal(nodal):
+    """coarse lattice value = sum_f P_fc v_f / sum_f P_fc (P trilinear, separable)"""
+    v = nodal
+    for ax in range(3):
+        v = _restrict_scaled_1d(v, ax)
+    return v
+
+
+def q1_interpolate_qp(mx, my, mz, nodal):
+    N = _q1_at_gauss()
+    out = np.zeros((mz, my, mx, 27))
+    cor = np.stack([nodal[2 * cz:2 * cz + 2 * mz:2, 2 * cy:2 * cy + 2 * my:2, 2 * cx:2 * cx + 2 * mx:2]
+                    for cz in range(2) for cy in range(2) for cx in range(2)], axis=-1)  # (ez, ey, ex, 8)
+    out = cor @ N.T
+    lo = np.full((mz, my, mx), np.inf)
+    hi = np.full((mz, my, mx), -np.inf)
+    for k in range(3):
+        for j in range(3):
+            for i in range(3):
+                v = nodal[k:k + 2 * mz:2, j:j + 2 * my:2, i:i + 2 * mx:2]
+                lo = np.minimum(lo, v)
+                hi = np.maximum(hi, v)
+    return np.clip(out, lo[..., None], hi[..., None]).reshape(-1)
+
+
+def q1_projection_hierarchy(mx, my, mz, eta, nlevels):
+    """coarse quadrature viscosities of levels 1..nlevels-1"""
+    nodal = q1_nodal_projection(mx, my, mz, eta)
+    out = []
+    for _ in range(nlevels - 1):
+        nodal = q1_restrict_nodal(nodal)
+        mx, my, mz = mx // 2, my // 2, mz // 2
+        out.append(q1_interpolate_qp(mx, my, mz, nodal))
+    return out
+
+
 def prolong(mxc, myc, mzc, xc):
     """trilinear interpolation of a coarse nodal 3-vector field to the fine lattice"""
     c = np.asarray(xc).reshape(2 * mzc + 1, 2 * myc + 1, 2 * mxc + 1, 3)
@@ -166,10 +268,12 @@ class Level:
 
 def hierarchy(mx, my, mz, coords, mask, eta, nlevels, mode="arithmetic"):
     lv = [Level(mx, my, mz, coords, mask, eta)]
-    for _ in range(nlevels - 1):
+    proj = q1_projection_hierarchy(mx, my, mz, eta, nlevels) if mode == "q1proj" else None
+    for l in range(nlevels - 1):
         f = lv[-1]
         mask = None if f.mask is None else inject_from_fine(f, f.mask)
-        c = Level(f.mx // 2, f.my // 2, f.mz // 2, inject_from_fine(f), mask, eta_coarsen(f.mx, f.my, f.mz, f.eta, mode))
+        ec = proj[l] if proj is not None else eta_coarsen(f.mx, f.my, f.mz, f.eta, mode)
+        c = Level(f.mx // 2, f.my // 2, f.mz // 2, inject_from_fine(f), mask, ec)
         lv.append(c)
     return lv
 

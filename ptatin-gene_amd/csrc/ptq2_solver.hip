@@ -218,6 +218,127 @@ __global__ void k_eta_coarsen(Box f, int mxc, int myc, int mzc, const double *__
   }
 }
 
+// natural-order quadrature viscosity (e*27+q) from the brick-major copy
+__global__ void k_eta_from_bm(Box f, const double *__restrict__ eta_bm, double *__restrict__ eta) {
+  const int64_t nq = 27 * (int64_t)f.mx * f.my * f.mz;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nq; t += (int64_t)gridDim.x * blockDim.x) {
+    const int q = (int)(t % 27);
+    const int64_t e = t / 27;
+    const int ex = (int)(e % f.mx), ey = (int)((e / f.mx) % f.my), ez = (int)(e / ((int64_t)f.mx * f.my));
+    const int64_t lin = (ex >> 2) + (int64_t)f.nbx * ((ey >> 2) + (int64_t)f.nby * (ez >> 2));
+    eta[t] = eta_bm[lin * 27 * ptq2::BEL + q * ptq2::BEL + ptq2::elem_lane(ex & 3, ey & 3, ez & 3)];
+  }
+}
+
+// ---- the reference's coefficient hierarchy (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_-
+// MPntPStokes_Hierarchy, src/material_point_utils.c:1325-1524, projection type 0), on the host at
+// set-up: a nodal field on the Q2 node lattice from the fine quadrature values, restricted lattice
+// by lattice, interpolated to each coarse level's quadrature points.
+
+// Q1 weight of corner c = cx + 2cy + 4cz at Gauss point p = i + 3j + 9k (element_utils_q1.c:32-47)
+double q1_gauss_weight(int p, int c) {
+  static const double g[3] = {-0.7745966692414834, 0.0, 0.7745966692414834};
+  const int ix[3] = {p % 3, (p / 3) % 3, p / 9};
+  double w = 0.125;
+  for (int a = 0; a < 3; a++) w *= 1.0 + (((c >> a) & 1) ? g[ix[a]] : -g[ix[a]]);
+  return w;
+}
+
+// _BuildQ1CoefficientProjection_QuadraturePoints_MPntPStokes_FineGrid (:1196-1300): per element and
+// point, weight N_c(xi_p) on the 8 corner nodes and 1 on the others; node = sum(w eta) / sum(w)
+void q1_nodal_projection(int mx, int my, int mz, const double *eta, std::vector<double> &nodal) {
+  const int NX = 2 * mx + 1, NY = 2 * my + 1, NZ = 2 * mz + 1;
+  std::vector<double> A((size_t)NX * NY * NZ, 0.0), B(A.size(), 0.0);
+  double W[27][8], wsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int p = 0; p < 27; p++)
+    for (int c = 0; c < 8; c++) {
+      W[p][c] = q1_gauss_weight(p, c);
+      wsum[c] += W[p][c];
+    }
+  for (int ez = 0; ez < mz; ez++)
+    for (int ey = 0; ey < my; ey++)
+      for (int ex = 0; ex < mx; ex++) {
+        const double *e = eta + 27 * ((int64_t)ex + (int64_t)mx * (ey + (int64_t)my * ez));
+        double sum = 0.0, cw[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int p = 0; p < 27; p++) {
+          sum += e[p];
+          for (int c = 0; c < 8; c++) cw[c] += W[p][c] * e[p];
+        }
+        for (int k = 0; k < 3; k++)
+          for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) {
+              const int64_t n = (int64_t)(2 * ex + i) + (int64_t)NX * ((2 * ey + j) + (int64_t)NY * (2 * ez + k));
+              if ((i & 1) || (j & 1) || (k & 1)) {
+                A[n] += sum;
+                B[n] += 27.0;
+              } else {
+                const int c = (i >> 1) + 2 * (j >> 1) + 4 * (k >> 1);
+                A[n] += cw[c];
+                B[n] += wsum[c];
+              }
+            }
+      }
+  nodal.resize(A.size());
+  for (size_t n = 0; n < A.size(); n++) nodal[n] = A[n] / B[n];
+}
+
+// MatRestrict with the trilinear lattice interpolation, scaled by 1 / (R 1): along each axis a
+// fine node 2I weighs 1 and its odd neighbours 1/2, normalised by the weights present
+void q1_restrict_nodal(int mxc, int myc, int mzc, const std::vector<double> &f, std::vector<double> &c) {
+  const int NXc = 2 * mxc + 1, NYc = 2 * myc + 1, NZc = 2 * mzc + 1;
+  const int NXf = 2 * NXc - 1, NYf = 2 * NYc - 1;
+  c.assign((size_t)NXc * NYc * NZc, 0.0);
+  for (int K = 0; K < NZc; K++)
+    for (int J = 0; J < NYc; J++)
+      for (int I = 0; I < NXc; I++) {
+        double num = 0.0, den = 0.0;
+        for (int dk = -1; dk <= 1; dk++) {
+          const int fk = 2 * K + dk;
+          if (fk < 0 || fk > 2 * (NZc - 1)) continue;
+          const double wk = dk ? 0.5 : 1.0;
+          for (int dj = -1; dj <= 1; dj++) {
+            const int fj = 2 * J + dj;
+            if (fj < 0 || fj > 2 * (NYc - 1)) continue;
+            const double wj = dj ? 0.5 : 1.0;
+            for (int di = -1; di <= 1; di++) {
+              const int fi = 2 * I + di;
+              if (fi < 0 || fi > 2 * (NXc - 1)) continue;
+              const double w = wk * wj * (di ? 0.5 : 1.0);
+              num += w * f[(size_t)fi + (size_t)NXf * (fj + (size_t)NYf * fk)];
+              den += w;
+            }
+          }
+        }
+        c[(size_t)I + (size_t)NXc * (J + (size_t)NYc * K)] = num / den;
+      }
+}
+
+// _SwarmUpdateGaussPropertiesLocalL2ProjectionQ1_MPntPStokes_InterpolateToQuadratePoints
+// (:957-1070): Q1 interpolation of the corner nodes, clipped to the element's nodal range
+void q1_interpolate_qp(int mx, int my, int mz, const std::vector<double> &nodal, std::vector<double> &eta) {
+  const int NX = 2 * mx + 1, NY = 2 * my + 1;
+  eta.resize((size_t)27 * mx * my * mz);
+  for (int ez = 0; ez < mz; ez++)
+    for (int ey = 0; ey < my; ey++)
+      for (int ex = 0; ex < mx; ex++) {
+        double cv[8], lo = 1e300, hi = -1e300;
+        for (int k = 0; k < 3; k++)
+          for (int j = 0; j < 3; j++)
+            for (int i = 0; i < 3; i++) {
+              const double v = nodal[(size_t)(2 * ex + i) + (size_t)NX * ((2 * ey + j) + (size_t)NY * (2 * ez + k))];
+              lo = std::min(lo, v);
+              hi = std::max(hi, v);
+              if (!((i & 1) || (j & 1) || (k & 1))) cv[(i >> 1) + 2 * (j >> 1) + 4 * (k >> 1)] = v;
+            }
+        double *out = &eta[27 * ((size_t)ex + (size_t)mx * (ey + (size_t)my * ez))];
+        for (int p = 0; p < 27; p++) {
+          double s = 0.0;
+          for (int c = 0; c < 8; c++) s += q1_gauss_weight(p, c) * cv[c];
+          out[p] = std::min(std::max(s, lo), hi);
+        }
+      }
+}
+
 // Dot products and updates of the Gram-Schmidt passes.  The reductions run on a fixed
 // grid of MDB blocks: every block writes one partial per product and k_sum_parts adds a
 // product's partials in block order, so every sum is reproducible run to run.
@@ -1522,7 +1643,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   if (opts) o = *opts;
   else ptq2_mg_default_opts(&o);
   if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0) ||
-      o.coarse_eta_mean < PTQ2_ETA_MEAN_ARITHMETIC || o.coarse_eta_mean > PTQ2_ETA_MEAN_MAX)
+      o.coarse_eta_mean < PTQ2_ETA_MEAN_ARITHMETIC || o.coarse_eta_mean > PTQ2_ETA_MEAN_Q1PROJ)
     return PTQ2_ERR_INVALID_ARG;
   if (o.rigid_deflation &&
       (!(o.rigid_min_contrast >= 1.0) || o.rigid_max_inclusions < 1 || o.rigid_max_inclusions > RG_MAXINC))
@@ -1554,6 +1675,20 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   std::vector<uint8_t> mf((size_t)fine->nvel, 0);
   MGCHK(hipMemcpy(cf.data(), fine->coords, sizeof(double) * fine->nvel, hipMemcpyDeviceToHost));
   if (fine->mask) MGCHK(hipMemcpy(mf.data(), fine->mask, fine->nvel, hipMemcpyDeviceToHost));
+  std::vector<double> q1nodal;  // lattice field of the reference's coefficient hierarchy
+  if (o.coarse_eta_mean == PTQ2_ETA_MEAN_Q1PROJ) {
+    std::vector<double> ef((size_t)27 * fine->nel);
+    double *d = nullptr;
+    MGRC(mg_alloc(mg, (void **)&d, sizeof(double) * ef.size()));
+    hipLaunchKernelGGL(k_eta_from_bm, dim3(nblk(27 * fine->nel)), dim3(TPB), 0, mg->stream, fine->box,
+                       (const double *)fine->eta_bm, d);
+    hipError_t e1 = hipMemcpyAsync(ef.data(), d, sizeof(double) * ef.size(), hipMemcpyDeviceToHost, mg->stream);
+    hipError_t e2 = hipStreamSynchronize(mg->stream);
+    (void)hipFree(d);
+    MGCHK(e1);
+    MGCHK(e2);
+    q1_nodal_projection(fine->box.mx, fine->box.my, fine->box.mz, ef.data(), q1nodal);
+  }
   for (int l = 1; l < mg->nl; l++) {
     const int *pm = &mg->m[3 * (l - 1)];
     int mx = pm[0] / 2, my = pm[1] / 2, mz = pm[2] / 2;
@@ -1577,15 +1712,25 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     mg->owned.push_back(true);
     ptq2_set_stream(c, mg->stream);
     MGRC(ptq2_setup_mesh(c, mx, my, mz, cc.data(), nullptr, fine->mask ? mc.data() : nullptr));
-    // viscosity from the finer level
-    double *etac = nullptr;
-    MGRC(mg_alloc(mg, (void **)&etac, sizeof(double) * 27 * (size_t)c->nel));
-    ptq2_ctx *p = mg->ctx[l - 1];
-    hipLaunchKernelGGL(k_eta_coarsen, dim3(nblk(27 * c->nel)), dim3(TPB), 0, mg->stream, p->box, mx, my, mz,
-                       (const double *)p->eta_bm, o.coarse_eta_mean, etac);
-    int rc = ptq2_set_viscosity(c, etac, 0, PTQ2_DEVICE);
-    (void)hipFree(etac);
-    MGRC(rc);
+    if (o.coarse_eta_mean == PTQ2_ETA_MEAN_Q1PROJ) {
+      // the reference's hierarchy: the lattice field of the level above, restricted, then
+      // interpolated to this level's quadrature points
+      std::vector<double> nc, ec;
+      q1_restrict_nodal(mx, my, mz, q1nodal, nc);
+      q1_interpolate_qp(mx, my, mz, nc, ec);
+      q1nodal.swap(nc);
+      MGRC(ptq2_set_viscosity(c, ec.data(), 0, 0));
+    } else {
+      // viscosity from the finer level
+      double *etac = nullptr;
+      MGRC(mg_alloc(mg, (void **)&etac, sizeof(double) * 27 * (size_t)c->nel));
+      ptq2_ctx *p = mg->ctx[l - 1];
+      hipLaunchKernelGGL(k_eta_coarsen, dim3(nblk(27 * c->nel)), dim3(TPB), 0, mg->stream, p->box, mx, my, mz,
+                         (const double *)p->eta_bm, o.coarse_eta_mean, etac);
+      int rc = ptq2_set_viscosity(c, etac, 0, PTQ2_DEVICE);
+      (void)hipFree(etac);
+      MGRC(rc);
+    }
     mg->m.insert(mg->m.end(), {mx, my, mz});
     mg->n.push_back(c->nvel);
     cf.swap(cc);

@@ -20,7 +20,7 @@ class MGOpts(ctypes.Structure):
                 ("coarse_eta_mean", ctypes.c_int), ("fp32_basis", ctypes.c_int), ("rigid_deflation", ctypes.c_int),
                 ("rigid_min_contrast", ctypes.c_double), ("rigid_max_inclusions", ctypes.c_int)]
 
-ETA_MEANS = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "max": 3}
+ETA_MEANS = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "max": 3, "q1proj": 4}
 
 
 _SOLVER_SIGS = {

@@ -52,7 +52,7 @@ def test_vcycle_fp64_matches_restatement(galerkin):
     assert 0.9 * lam <= emax[0] / 1.1 <= 1.01 * lam
 
 
-@pytest.mark.parametrize("mean", ["geometric", "harmonic", "max"])
+@pytest.mark.parametrize("mean", ["geometric", "harmonic", "max", "q1proj"])
 def test_vcycle_coarse_viscosity_rules(mean):
     """The coarse quadrature viscosity rules other than the arithmetic default, through the whole
     V-cycle, against the restatement's hierarchy built with the same rule; then the V-cycle

@@ -58,7 +58,7 @@ def test_solver_default_options_round_trip():
     assert (o.emin_frac, o.emax_frac) == (0.2, 1.1)
     assert (o.coarse_max_dofs, o.galerkin_coarse, o.coarse_eta_mean) == (20000, 1, Sv.ETA_MEANS["arithmetic"])
     assert (o.fp32_basis, o.rigid_deflation, o.rigid_min_contrast, o.rigid_max_inclusions) == (0, 0, 100.0, 128)
-    assert sorted(Sv.ETA_MEANS.values()) == [0, 1, 2, 3]
+    assert sorted(Sv.ETA_MEANS.values()) == [0, 1, 2, 3, 4]
     txt = open(os.path.join(REPO, "include", "ptq2_solver.h")).read()
     for name, v in Sv.ETA_MEANS.items():
         assert re.search(r"PTQ2_ETA_MEAN_%s\s*=\s*%d" % (name.upper(), v), txt), name

@@ -197,3 +197,83 @@ def test_rigid_motions_have_no_strain_and_adef2_reproduces_them():
     Wd = np.concatenate([D.W, D.W[:, :1]], axis=1)
     E = Wd.T @ np.stack([L.matmult(Wd[:, j]) for j in range(7)], axis=1)
     assert G.drop_cholesky_keep(0.5 * (E + E.T)) == [0, 1, 2, 3, 4, 5]
+
+
+def _q1proj_brute(mx, my, mz, eta, nlevels):
+    """The reference's coefficient hierarchy written as plain loops (element by element, point by
+    point, node by node) for tiny meshes."""
+    g = [-np.sqrt(0.6), 0.0, np.sqrt(0.6)]
+    NX, NY, NZ = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+    A = np.zeros(NX * NY * NZ)
+    B = np.zeros(NX * NY * NZ)
+    e = np.asarray(eta).reshape(-1, 27)
+    for ez in range(mz):
+        for ey in range(my):
+            for ex in range(mx):
+                el = ex + mx * (ey + my * ez)
+                for p in range(27):
+                    xi = (g[p % 3], g[(p // 3) % 3], g[p // 9])
+                    for c in range(3):
+                        for b in range(3):
+                            for a in range(3):
+                                n = (2 * ex + a) + NX * ((2 * ey + b) + NY * (2 * ez + c))
+                                if a % 2 == 0 and b % 2 == 0 and c % 2 == 0:
+                                    w = 0.125
+                                    for s, x in zip((a, b, c), xi):
+                                        w *= (1 + x) if s == 2 else (1 - x)
+                                else:
+                                    w = 1.0
+                                A[n] += w * e[el, p]
+                                B[n] += w
+    nodal = (A / B).reshape(NZ, NY, NX)
+    out = []
+    for _ in range(nlevels - 1):
+        mx, my, mz = mx // 2, my // 2, mz // 2
+        cx, cy, cz = 2 * mx + 1, 2 * my + 1, 2 * mz + 1
+        c = np.zeros((cz, cy, cx))
+        for K in range(cz):
+            for J in range(cy):
+                for I in range(cx):
+                    num = den = 0.0
+                    for fk in range(nodal.shape[0]):
+                        for fj in range(nodal.shape[1]):
+                            for fi in range(nodal.shape[2]):
+                                w = 1.0
+                                for f, C in ((fk, K), (fj, J), (fi, I)):
+                                    w *= 1.0 if f == 2 * C else (0.5 if abs(f - 2 * C) == 1 else 0.0)
+                                num += w * nodal[fk, fj, fi]
+                                den += w
+                    c[K, J, I] = num / den
+        nodal = c
+        q = np.zeros((mx * my * mz, 27))
+        for ez in range(mz):
+            for ey in range(my):
+                for ex in range(mx):
+                    vals = nodal[2 * ez:2 * ez + 3, 2 * ey:2 * ey + 3, 2 * ex:2 * ex + 3]
+                    for p in range(27):
+                        xi = (g[p % 3], g[(p // 3) % 3], g[p // 9])
+                        s = 0.0
+                        for cc in range(8):
+                            w = 0.125
+                            for ax in range(3):
+                                w *= (1 + xi[ax]) if (cc >> ax) & 1 else (1 - xi[ax])
+                            s += w * vals[2 * (cc >> 2), 2 * ((cc >> 1) & 1), 2 * (cc & 1)]
+                        q[ex + mx * (ey + my * ez), p] = min(max(s, vals.min()), vals.max())
+        out.append(q.reshape(-1))
+    return out
+
+
+def test_q1_projection_hierarchy_matches_loops_and_invariants():
+    """The reference's coefficient hierarchy (coarse_eta_mean = q1proj): the vectorised
+    restatement against the plain-loop one; constant fields stay exact; coarse values stay
+    within the fine range (all weights are non-negative)."""
+    rng = np.random.default_rng(5)
+    eta = 10.0 ** rng.uniform(-1, 3, 4 * 2 * 4 * 27)
+    fast = G.q1_projection_hierarchy(4, 2, 4, eta, 2)
+    slow = _q1proj_brute(4, 2, 4, eta, 2)
+    assert np.allclose(fast[0], slow[0], rtol=1e-13, atol=0)
+    for lv in G.q1_projection_hierarchy(8, 8, 4, np.full(8 * 8 * 4 * 27, 2.5), 3):
+        assert np.allclose(lv, 2.5, rtol=1e-15, atol=0)
+    big = 10.0 ** rng.uniform(0, 6, 8 * 8 * 8 * 27)
+    for lv in G.q1_projection_hierarchy(8, 8, 8, big, 4):
+        assert lv.min() >= big.min() and lv.max() <= big.max()
