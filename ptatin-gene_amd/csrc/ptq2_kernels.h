@@ -13,11 +13,13 @@ constexpr int ERRBIT_NONFINITE = 4;
 
 // A11 launchers: with facebuf (FACEVALS values per brick) all bricks run in one launch,
 // storing the nodes they own, and k_gather_faces adds the shared face values into their
-// owners; without it, 8 coloured launches store/add directly.
+// owners (gather = false leaves that to the caller's next kernel, which then reads the face
+// slots itself); without facebuf, 8 coloured launches store/add directly.
 hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
-                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf = nullptr);
+                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf = nullptr,
+                           bool gather = true);
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
-                           hipStream_t s, float *facebuf = nullptr);
+                           hipStream_t s, float *facebuf = nullptr, bool gather = true);
 hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, float *facb,
                               int *err, hipStream_t s);
 hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,

@@ -1100,11 +1100,11 @@ hipError_t launch_gather(const Box &b, const T *face, T *y, hipStream_t s) {
 }
 
 hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
-                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf) {
+                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf, bool gather) {
   if (facebuf) {
     hipLaunchKernelGGL(k_a11_fp64, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm, y, add ? 1 : 0,
                        facebuf);
-    return launch_gather(b, (const double *)facebuf, y, s);
+    return gather ? launch_gather(b, (const double *)facebuf, y, s) : hipGetLastError();
   }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
@@ -1116,10 +1116,10 @@ hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, 
 }
 
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
-                           hipStream_t s, float *facebuf) {
+                           hipStream_t s, float *facebuf, bool gather) {
   if (facebuf) {
     hipLaunchKernelGGL(k_a11_fp32, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, fac, y, add ? 1 : 0, facebuf);
-    return launch_gather(b, (const float *)facebuf, y, s);
+    return gather ? launch_gather(b, (const float *)facebuf, y, s) : hipGetLastError();
   }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);

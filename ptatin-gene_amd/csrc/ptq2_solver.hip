@@ -84,13 +84,54 @@ __global__ void k_cheb_first(int64_t n, const T *__restrict__ b, const T *__rest
   }
 }
 
-// fused Chebyshev step: p_{k+1} = omega*scale*Dinv*(b - A p_k) + (1-omega) p_{k-1} + omega p_k
+// The Chebyshev updates fused with the face pass of the A11 apply before them: y holds the
+// apply's owner-written values and face the upper-face slots of every brick (FACEVALS per
+// brick, the layout k_gather_faces reads); a dof on a brick's lower face adds the slots of the
+// lower neighbours in k_gather_faces' order, so the result is bitwise that of the unfused
+// apply + update.  Dirichlet dofs take y = x (the apply input; MatMult semantics).  Then
+//   out = cr * dinv * (b - y) + ck * pkm1 + cx * x       (pkm1 may be null: first step).
+// The strided read-modify-write of the x faces that dominates the separate pass goes away.
 template <typename T>
-__global__ void k_cheb_step(int64_t n, const T *__restrict__ b, const T *__restrict__ y, const T *__restrict__ dinv,
-                            const T *__restrict__ pkm1, const T *__restrict__ pk, T *__restrict__ pkp1, T os, T om1,
-                            T om) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    pkp1[i] = os * dinv[i] * (b[i] - y[i]) + om1 * pkm1[i] + om * pk[i];
+__global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T *__restrict__ y,
+                             const T *__restrict__ dinv, const T *__restrict__ x, const T *__restrict__ pkm1,
+                             T *__restrict__ out, const T *__restrict__ face, const uint8_t *__restrict__ mask, T cr,
+                             T ck, T cx) {
+  constexpr int TN = ptq2::TN, F2 = TN * TN, UP = 2 * ptq2::BE, NB = 2 * ptq2::BE;
+  const unsigned NX = (unsigned)bx.NX, NXY = (unsigned)bx.NX * (unsigned)bx.NY;
+  const int64_t sx = ptq2::FACEVALS, sy = (int64_t)bx.nbx * ptq2::FACEVALS,
+                sz = (int64_t)bx.nbx * bx.nby * ptq2::FACEVALS;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    T yv;
+    if (mask && mask[i]) {
+      yv = x[i];
+    } else {
+      yv = y[i];
+      if (face) {
+        const unsigned nd = (unsigned)(i / 3), c = (unsigned)(i - 3 * (int64_t)nd);
+        const unsigned Z = nd / NXY, rem = nd - Z * NXY, Yn = rem / NX, X = rem - Yn * NX;
+        const int bxi = min((int)(X / NB), bx.nbx - 1), byi = min((int)(Yn / NB), bx.nby - 1),
+                  bzi = min((int)(Z / NB), bx.nbz - 1);
+        const int lx = (int)X - NB * bxi, ly = (int)Yn - NB * byi, lz = (int)Z - NB * bzi;
+        const bool nx = lx == 0 && bxi > 0, ny = ly == 0 && byi > 0, nz = lz == 0 && bzi > 0;
+        if (nx || ny || nz) {
+          const T *fb = face + ((int64_t)bxi + (int64_t)bx.nbx * (byi + (int64_t)bx.nby * bzi)) * ptq2::FACEVALS;
+          T acc = yv;
+          if (nz && ny && nx) acc += fb[-sx - sy - sz + 3 * (UP * TN + UP) + c];
+          if (nz && ny) acc += fb[-sy - sz + 3 * (F2 + UP * TN + lx) + c];
+          if (nz && nx) acc += fb[-sx - sz + 3 * (UP * TN + ly) + c];
+          if (nz) acc += fb[-sz + 3 * (2 * F2 + ly * TN + lx) + c];
+          if (ny && nx) acc += fb[-sx - sy + 3 * (lz * TN + UP) + c];
+          if (ny) acc += fb[-sy + 3 * (F2 + lz * TN + lx) + c];
+          if (nx) acc += fb[-sx + 3 * (lz * TN + ly) + c];
+          yv = acc;
+        }
+      }
+    }
+    if (pkm1)
+      out[i] = cr * dinv[i] * (b[i] - yv) + ck * pkm1[i] + cx * x[i];
+    else
+      out[i] = cr * dinv[i] * (b[i] - yv) + x[i];
+  }
 }
 
 // restriction R = P^T of the residual b - y onto the coarse node lattice, zero on
@@ -1002,6 +1043,23 @@ int level_matmult<float>(ptq2_mg *mg, int l, const float *x, float *y) {
   return PTQ2_OK;
 }
 
+// the level's A11 apply without its face pass and Dirichlet rows: k_cheb_faces completes y
+template <typename T>
+int level_apply_owned(ptq2_mg *mg, int l, const T *x, T *y);
+template <>
+int level_apply_owned<double>(ptq2_mg *mg, int l, const double *x, double *y) {
+  ptq2_ctx *c = mg->ctx[l];
+  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream, (double *)c->facebuf,
+                              false));
+  return PTQ2_OK;
+}
+template <>
+int level_apply_owned<float>(ptq2_mg *mg, int l, const float *x, float *y) {
+  ptq2_ctx *c = mg->ctx[l];
+  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, false));
+  return PTQ2_OK;
+}
+
 // ChebychevRN on level l: x <- smoothed solution of A x = b; zero_guess skips the first residual
 template <typename T>
 int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
@@ -1016,22 +1074,26 @@ int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
   int km1 = 0, k = 1, kp1 = 2;
   c[km1] = 1.0;
   c[k] = mu;
+  // apply (owner values + face slots) and update in one pass over the vectors
+  const ptq2_ctx *lc = mg->ctx[l];
+  auto fused = [&](const T *xin, const T *pkm1, T *out, double cr, double ck, double cx) {
+    hipLaunchKernelGGL(k_cheb_faces<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, lc->box, n, (const T *)b,
+                       (const T *)y, (const T *)dinv, xin, pkm1, out, (const T *)lc->facebuf,
+                       (const uint8_t *)lc->mask, (T)cr, (T)ck, (T)cx);
+  };
   if (zero_guess) {
     hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const T *)b, (const T *)nullptr,
                        (const T *)dinv, (const T *)nullptr, buf[k], (T)scale);
     hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, x, T(0));
   } else {
-    MGRC(level_matmult<T>(mg, l, x, y));
-    hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const T *)b, (const T *)y,
-                       (const T *)dinv, (const T *)x, buf[k], (T)scale);
+    MGRC(level_apply_owned<T>(mg, l, x, y));
+    fused(x, nullptr, buf[k], scale, 0.0, 1.0);
   }
   for (int i = 0; i < mg->opt.smooth_its; i++) {
     c[kp1] = 2.0 * mu * c[k] - c[km1];
     const double omega = omegaprod * c[k] / c[kp1];
-    MGRC(level_matmult<T>(mg, l, buf[k], y));
-    hipLaunchKernelGGL(k_cheb_step<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const T *)b, (const T *)y,
-                       (const T *)dinv, (const T *)buf[km1], (const T *)buf[k], buf[kp1], (T)(omega * scale),
-                       (T)(1.0 - omega), (T)omega);
+    MGRC(level_apply_owned<T>(mg, l, buf[k], y));
+    fused(buf[k], buf[km1], buf[kp1], omega * scale, 1.0 - omega, omega);
     int t = km1;
     km1 = k;
     k = kp1;
