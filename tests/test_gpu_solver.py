@@ -268,3 +268,32 @@ def test_rigid_deflation_speeds_up_high_contrast_solve():
         assert np.linalg.norm(r) <= 1.05e-8 * np.linalg.norm(b)
         S.destroy()
     assert its[True] < 0.8 * its[False]
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_vcycle_non_cubic_partial_bricks(fp32):
+    """V-cycle on a 12x8x20 jittered mesh with free-slip walls and log-uniform viscosity: level 1
+    (6x4x10) has partial bricks in x and z, so the smoother's fused face pass meets bricks of
+    every size; FP64 against the restatement at 1e-10, FP32 smoothing at 1e-4."""
+    mx, my, mz = 12, 8, 20
+    say("case: V-cycle %dx%dx%d, 3 levels, fp32=%s" % (mx, my, mz, fp32))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * mz), 5)
+    mask = M.bc_mask(mx, my, mz, "freeslip")
+    eta = 10.0 ** M.uniform(6, 27 * mx * my * mz, -1.0, 2.0)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    ctx.set_viscosity(eta)
+    S = ViscousSolver(ctx, nlevels=3, smooth_its=3, fp32=fp32, use_graph=True, galerkin_coarse=False)
+    assert [tuple(r) for r in S.levels()] == [(12, 8, 20), (6, 4, 10), (3, 2, 5)]
+    emin, emax = S.eigs()
+    lv = G.hierarchy(mx, my, mz, c, mask, eta, 3)
+    V = G.VCycle(lv, emin, emax, 3)
+    r = M.uniform(7, c.size, -1, 1)
+    r[mask > 0] = 0.0
+    z = S.vcycle(torch.from_numpy(r).to(DEV)).cpu().numpy()
+    zr = V(r)
+    err = np.linalg.norm(z - zr) / np.linalg.norm(zr)
+    say("  rel err %.3e" % err)
+    assert err <= (1e-4 if fp32 else 1e-10)
+    S.destroy()
+    ctx.destroy()
