@@ -75,7 +75,17 @@ enum {
    operator, arithmetic coarse viscosity */
 void ptq2_mg_default_opts(ptq2_mg_opts *opts);
 
-/* Build the hierarchy under a set-up context (mesh with BC mask + viscosity): coarse
+/* Ownership: from ptq2_mg_create to ptq2_mg_destroy the hierarchy owns the fine context.  Its
+   captured V-cycle holds the fine context's device buffers (coordinates, mask, viscosity,
+   factors, face slots) and its coarse levels were derived from the fine viscosity, so a new
+   ptq2_setup_mesh* or ptq2_set_viscosity upload on the fine context makes ptq2_mg_vcycle and
+   ptq2_mg_solve return PTQ2_ERR_NOT_SETUP (rebuild the hierarchy).  The hierarchy moves the
+   fine context onto its own stream (ptq2_mg_set_stream moves both); operator calls on the fine
+   context must stay on that stream, since they share its face-slot buffer with the V-cycle.
+   Device-pointer calls do not scan their vectors element by element: ptq2_mg_solve rejects a
+   right-hand side whose norm is not finite (PTQ2_ERR_NONFINITE).
+
+   Build the hierarchy under a set-up context (mesh with BC mask + viscosity): coarse
    meshes by coordinate injection, BC masks by injection, quadrature-point viscosity
    by averaging over the 8 children of each coarse element, Jacobi diagonals,
    eigen-estimates, the dense FP64 inverse of the coarsest operator, with rigid_deflation the
@@ -110,6 +120,9 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
 
 /* residual history of the last solve (one norm per iteration, up to cap entries) */
 int ptq2_mg_history(const ptq2_mg *mg, double *hist, int cap, int *n);
+
+/* message of the last failing call on this hierarchy ("" when none) */
+const char *ptq2_mg_last_error(const ptq2_mg *mg);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,8 @@ def lib():
             "ptq2o_matgetdiag_a11": [_i, _i, _i, _D, _D, _U8, _D],
             "ptq2o_matmult_a": [_i, _i, _i, _D, _D, _U8, _D, _D, _D, _D],
             "ptq2o_a11_element_matrix": [_D, _D, _D],
+            "ptq2o_residual": [_i, _i, _i, _D, _D, _U8, ctypes.c_void_p, _D, _D, _D, ctypes.c_void_p, _D, _D],
+            "ptq2o_body_force": [_i, _i, _i, _D, _D, _D],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -181,7 +183,55 @@ def matmult_a(mx, my, mz, coords, eta, mask, u, p):
     return yu, yp
 
 
+def residual(mx, my, mz, coords, eta, mask, u, p, fu, fp=None, phi=None):
+    """FormFunction_Stokes on one rank: (Fu, Fp) with Dirichlet values phi (None = 0) inserted
+    into u, body forces fu (E x 27 x 3) and fp (E x 27, None = 0), Fu_D = u_D - phi_D."""
+    n = 3 * nnodes(mx, my, mz)
+    Fu = np.zeros(n)
+    Fp = np.zeros(4 * mx * my * mz)
+    fpa = None if fp is None else _f64(fp)
+    pha = None if phi is None else _f64(phi)
+    lib().ptq2o_residual(mx, my, mz, _f64(coords), _f64(eta), _mask(mask, n),
+                         None if pha is None else pha.ctypes.data, _f64(u), _f64(p), _f64(fu),
+                         None if fpa is None else fpa.ctypes.data, Fu, Fp)
+    return Fu, Fp
+
+
+def body_force(mx, my, mz, coords, fu):
+    """sum_e sum_q w detJ N fu (fu: E x 27 x 3), per quadrature point"""
+    y = np.zeros(3 * nnodes(mx, my, mz))
+    lib().ptq2o_body_force(mx, my, mz, _f64(coords), _f64(fu), y)
+    return y
+
+
 def a11_element_matrix(elcoords, eta27):
     K = np.zeros(81 * 81)
     lib().ptq2o_a11_element_matrix(_f64(elcoords), _f64(eta27), K)
     return K.reshape(81, 81)
+
+
+# ---- CPU baseline: the restatement of the reference's default AVX A11 backend (ptq2_avx.c)
+_AVX = os.path.join(_HERE, "libptq2_avx.so")
+_avx = None
+
+
+def avx_lib():
+    global _avx
+    if _avx is None:
+        if not os.path.exists(_AVX):
+            build()
+        L = ctypes.CDLL(_AVX)
+        L.ptq2a_a11_avx.argtypes = [_i, _i, _i, _D, _D, _D, _D, _i]
+        L.ptq2a_a11_avx.restype = None
+        L.ptq2a_flops_per_element.argtypes = []
+        L.ptq2a_flops_per_element.restype = ctypes.c_double
+        _avx = L
+    return _avx
+
+
+def a11_avx(mx, my, mz, coords, eta, u, threads=1, y=None):
+    """y += A11 u by the AVX NEV=4 restatement (MFStokesWrapper_A11_AVX) on `threads` cores."""
+    if y is None:
+        y = np.zeros(3 * nnodes(mx, my, mz))
+    avx_lib().ptq2a_a11_avx(mx, my, mz, _f64(coords), _f64(eta), _f64(u), y, int(threads))
+    return y

@@ -681,3 +681,91 @@ void ptq2o_matmult_a(int mx, int my, int mz, const double *coords, const double 
       if (mask[i]) yu[i] = u[i];
   free(xl);
 }
+
+/* ------------------------------------------------------------- residual */
+
+/* FormFunction_Stokes (src/stokes_form_function.c:478-580) on one rank with the
+   per-quadrature-point element loops of FormFunctionLocal_U (:99-242) and FormFunctionLocal_P
+   (:365-463):
+     u_loc = u with the Dirichlet values phi inserted (BCListInsertLocal, src/dmda_bcs.c:547-573);
+     Fu = sum_e [ int dN . (2 eta eps(u_loc) - p I) - int N fu ]      (fu = QPntVolCoefStokes.Fu)
+     Fp = sum_e [ -int N^p div u_loc - int N^p fp ]                  (fp = QPntVolCoefStokes.Fp)
+     Fu_D = u_D - phi_D   (BCListResidualDirichlet, src/dmda_bcs.c:614-648, scale 1).
+   fu: 81 per element ([e][q][d]), fp: 27 per element or NULL (0), phi: 3N or NULL (0). */
+void ptq2o_residual(int mx, int my, int mz, const double *coords, const double *eta, const uint8_t *mask,
+                    const double *phi, const double *u, const double *p, const double *fu, const double *fp,
+                    double *Fu, double *Fp) {
+  static double W[NQP], XI[NQP][3], NI[NQP][27], GNI[NQP][3][27];
+  prepare_q2_3x3(W, XI, NI, GNI);
+  const int64_t n = nvel(mx, my, mz), nel = (int64_t)mx * my * mz;
+  double *ul = (double *)malloc(sizeof(double) * n);
+  for (int64_t i = 0; i < n; i++) {
+    ul[i] = (mask && mask[i]) ? (phi ? phi[i] : 0.0) : u[i];
+    Fu[i] = 0.0;
+  }
+  for (int64_t e = 0; e < nel; e++) {
+    double elx[3][27], elu[3][27], Fe[3][27], Be[3][27], Fpe[4] = {0, 0, 0, 0};
+    gather_element(mx, my, (int)e, coords, elx);
+    gather_element(mx, my, (int)e, ul, elu);
+    memset(Fe, 0, sizeof Fe);
+    memset(Be, 0, sizeof Be);
+    for (int q = 0; q < NQP; q++) {
+      double detJ, dN[3][27], Np[4];
+      geometry_qp(GNI[q], elx, &detJ, dN);
+      pressure_basis(XI[q], elx, Np);
+      const double fac = W[q] * detJ, et = eta[27 * e + q];
+      double pq = 0.0;
+      for (int i = 0; i < 4; i++) pq += Np[i] * p[4 * e + i];
+      pq *= fac;
+      double exx = 0, eyy = 0, ezz = 0, exy = 0, exz = 0, eyz = 0;
+      for (int k = 0; k < 27; k++) {
+        exx += dN[0][k] * elu[0][k];
+        eyy += dN[1][k] * elu[1][k];
+        ezz += dN[2][k] * elu[2][k];
+        exy += dN[1][k] * elu[0][k] + dN[0][k] * elu[1][k];
+        exz += dN[2][k] * elu[0][k] + dN[0][k] * elu[2][k];
+        eyz += dN[2][k] * elu[1][k] + dN[1][k] * elu[2][k];
+      }
+      const double d1 = 2.0 * et * fac, d2 = et * fac;
+      const double sxx = d1 * exx, syy = d1 * eyy, szz = d1 * ezz, sxy = d2 * exy, sxz = d2 * exz, syz = d2 * eyz;
+      for (int k = 0; k < 27; k++) {
+        Fe[0][k] += dN[0][k] * (sxx - pq) + dN[1][k] * sxy + dN[2][k] * sxz;
+        Fe[1][k] += dN[1][k] * (syy - pq) + dN[0][k] * sxy + dN[2][k] * syz;
+        Fe[2][k] += dN[2][k] * (szz - pq) + dN[0][k] * sxz + dN[1][k] * syz;
+        for (int d = 0; d < 3; d++) Be[d][k] += fac * NI[q][k] * fu[81 * e + 3 * q + d];
+      }
+      const double div = -(exx + eyy + ezz) * fac;
+      const double f = fp ? fp[27 * e + q] : 0.0;
+      for (int i = 0; i < 4; i++) Fpe[i] += Np[i] * div - fac * Np[i] * f;
+    }
+    for (int d = 0; d < 3; d++)
+      for (int k = 0; k < 27; k++) Fe[d][k] -= Be[d][k];
+    scatter_element(mx, my, (int)e, Fe, Fu);
+    for (int i = 0; i < 4; i++) Fp[4 * e + i] = Fpe[i];
+  }
+  if (mask)
+    for (int64_t i = 0; i < n; i++)
+      if (mask[i]) Fu[i] = u[i] - (phi ? phi[i] : 0.0);
+  free(ul);
+}
+
+/* The Be loop of FormFunctionLocal_U alone (src/stokes_form_function.c:219-224): y += sum_e
+   sum_q w detJ N fu, per quadrature point (the checker of the device body-force load). */
+void ptq2o_body_force(int mx, int my, int mz, const double *coords, const double *fu, double *y) {
+  static double W[NQP], XI[NQP][3], NI[NQP][27], GNI[NQP][3][27];
+  prepare_q2_3x3(W, XI, NI, GNI);
+  const int64_t nel = (int64_t)mx * my * mz;
+  for (int64_t e = 0; e < nel; e++) {
+    double elx[3][27], Be[3][27];
+    gather_element(mx, my, (int)e, coords, elx);
+    memset(Be, 0, sizeof Be);
+    for (int q = 0; q < NQP; q++) {
+      double detJ, dN[3][27];
+      geometry_qp(GNI[q], elx, &detJ, dN);
+      const double fac = W[q] * detJ;
+      for (int k = 0; k < 27; k++)
+        for (int d = 0; d < 3; d++) Be[d][k] += fac * NI[q][k] * fu[81 * e + 3 * q + d];
+    }
+    scatter_element(mx, my, (int)e, Be, y);
+  }
+}

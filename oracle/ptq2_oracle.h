@@ -67,6 +67,13 @@ void ptq2o_matmult_a(int mx, int my, int mz, const double *coords, const double 
 void ptq2o_a11_element_matrix(const double *elcoords /*81*/, const double *eta /*27*/,
                               double *Ke /*81*81, row = 3*node + comp*/);
 
+/* FormFunction_Stokes residual (momentum Fu 3N, continuity Fp 4E); fp and phi may be NULL */
+void ptq2o_residual(int mx, int my, int mz, const double *coords, const double *eta, const uint8_t *mask,
+                    const double *phi, const double *u, const double *p, const double *fu, const double *fp,
+                    double *Fu, double *Fp);
+/* y += body-force load of fu (81 per element) */
+void ptq2o_body_force(int mx, int my, int mz, const double *coords, const double *fu, double *y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -261,6 +261,7 @@ int ptq2_setup_mesh_box(ptq2_ctx *c, int mx, int my, int mz, int NX, int NY, int
 
   // release a previous mesh
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->gen++;
   dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
   dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->facebuf);
   c->coords = nullptr; c->mask = nullptr; c->dir_idx = nullptr; c->eta_bm = nullptr;
@@ -352,6 +353,7 @@ int ptq2_set_viscosity(ptq2_ctx *c, const double *eta, uint64_t state, unsigned 
   if (e != hipSuccess) return fail(c, PTQ2_ERR_DEVICE, "eta permutation: %s", hipGetErrorString(e));
   c->fac32_valid = false;
   c->facb32_valid = false;
+  c->gen++;
   if (bits & ptq2::ERRBIT_VISCOSITY) {
     c->eta_ready = false;
     return fail(c, PTQ2_ERR_VISCOSITY, "eta <= 0 or non-finite at a quadrature point");

@@ -27,6 +27,10 @@ struct ptq2_ctx {
   bool facb32_valid = false;
   bool has_state = false;
   uint64_t state = 0;
+  // bumped by every mesh set-up and every viscosity upload: a multigrid hierarchy built on this
+  // context records it and refuses to run once it has changed (its captured graph holds the
+  // device buffers of that mesh, and its coarse levels were derived from that viscosity)
+  uint64_t gen = 0;
   int *err = nullptr;
   // output path of the A11 applies: all bricks in one launch, each storing the nodes it owns
   // and its upper-face values into its slot of facebuf (FACEVALS doubles per brick), then a

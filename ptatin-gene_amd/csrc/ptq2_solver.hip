@@ -984,6 +984,7 @@ struct ptq2_mg {
   std::vector<double> hist;
   std::string last;
   void *fine_stream0 = nullptr;  // the fine context's stream before the hierarchy was built
+  uint64_t fine_gen = 0;         // the fine context's mesh/viscosity generation at set-up
   // rigid-body deflation of stiff inclusions (rg_nc == 0: off or none found)
   int rg_nc = 0, rg_n6 = 0, rg_kept = 0, rg_nchunk = 0;
   int64_t rg_nd = 0;
@@ -1267,6 +1268,48 @@ __global__ void k_pad_identity(int nc, int np, double *__restrict__ A) {
   if (idx >= (int64_t)np * np) return;
   int i = (int)(idx / np), j = (int)(idx % np);
   if (i >= nc || j >= nc) A[idx] = i == j ? 1.0 : 0.0;
+}
+// rows of coarse Dirichlet dofs: the unit row (the MatMult row y_D = x_D)
+__global__ void k_dirichlet_unit_rows(int64_t n, const uint8_t *__restrict__ mask, double *__restrict__ A, int ld) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && mask && mask[i]) A[i * ld + i] = 1.0;
+}
+// the dof of the probed colour class (node indices = (cx,cy,cz) modulo pp on each axis,
+// component comp) nearest to node (I,J,K): offsets -pr..pr per axis, -1 if outside the box
+__device__ __forceinline__ int64_t probe_member(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
+                                                int I, int J, int K) {
+  const int pr = pp >> 1;
+  int d = (cx - I % pp + pp) % pp, Ij = I + (d <= pr ? d : d - pp);
+  d = (cy - J % pp + pp) % pp;
+  int Jj = J + (d <= pr ? d : d - pp);
+  d = (cz - K % pp + pp) % pp;
+  int Kj = K + (d <= pr ? d : d - pp);
+  if (Ij < 0 || Ij >= NX || Jj < 0 || Jj >= NY || Kj < 0 || Kj >= NZ) return -1;
+  return 3 * ((int64_t)Ij + (int64_t)NX * ((int64_t)Jj + (int64_t)NY * Kj)) + comp;
+}
+// probe vector of a colour class: 1 on its non-Dirichlet dofs, 0 elsewhere
+__global__ void k_probe_vec(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
+                            const uint8_t *__restrict__ mask, double *__restrict__ e) {
+  const int64_t n = 3 * (int64_t)NX * NY * NZ;
+  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < n; d += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = d / 3;
+    const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / ((int64_t)NX * NY));
+    const bool on = (int)(d - 3 * node) == comp && I % pp == cx && J % pp == cy && K % pp == cz;
+    e[d] = on && !(mask && mask[d]) ? 1.0 : 0.0;
+  }
+}
+// entry r of the probe result belongs to the column of the class member next to dof r's node:
+// row j of A holds column j of the operator (A[j][r] = (A e_j)[r])
+__global__ void k_probe_scatter(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
+                                const uint8_t *__restrict__ mask, const double *__restrict__ w, double *__restrict__ A,
+                                int ld) {
+  const int64_t n = 3 * (int64_t)NX * NY * NZ;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = r / 3;
+    const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / ((int64_t)NX * NY));
+    const int64_t j = probe_member(NX, NY, NZ, pp, cx, cy, cz, comp, I, J, K);
+    if (j >= 0 && !(mask && mask[j])) A[j * ld + r] = w[r];
+  }
 }
 // symmetrise the round-off of the assembled operator
 __global__ void k_symmetrise(int np, double *__restrict__ A) {
@@ -1718,6 +1761,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   if (!mg) return PTQ2_ERR_ALLOC;
   std::unique_ptr<ptq2_mg, int (*)(ptq2_mg *)> guard(mg, [](ptq2_mg *p) { return ptq2_mg_destroy(&p); });
   mg->opt = o;
+  mg->fine_gen = fine->gen;
   mg->nl = o.nlevels;
   mg->f32 = o.fp32_smoother != 0;
   mg->basis32 = o.fp32_basis != 0;
@@ -1889,41 +1933,60 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     mg->ainv_ld = np;
     MGRC(mg_alloc(mg, (void **)&e, sizeof(double) * nc));
     int rc = PTQ2_OK;
+    double *w = nullptr;
+    MGRC(mg_alloc(mg, (void **)&w, sizeof(double) * nc));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nn)), dim3(TPB), 0, mg->stream, nn, mg->ainv, 0.0);
     hipLaunchKernelGGL(k_pad_identity, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, nc, np,
                        mg->ainv);
-    if (mg->opt.galerkin_coarse && mg->nl >= 2) {
-      // row j = Pi R A_f P Pi e_j (Pi zeroes coarse Dirichlet dofs, whose rows are e_j): the
-      // Galerkin product through the level above, probed one coarse dof at a time
-      const int lf = l - 1;
-      const int *mf = &mg->m[3 * lf], *mcc = &mg->m[3 * l];
-      const int NXf = 2 * mf[0] + 1, NYf = 2 * mf[1] + 1, NZf = 2 * mf[2] + 1;
-      const int NXc = 2 * mcc[0] + 1, NYc = 2 * mcc[1] + 1, NZc = 2 * mcc[2] + 1;
-      const int64_t nf = mg->n[lf];
-      std::vector<uint8_t> hm((size_t)nc, 0);
-      if (c->mask) MGCHK(hipMemcpy(hm.data(), c->mask, nc, hipMemcpyDeviceToHost));
-      double *vf = nullptr, *wf = nullptr;
+    const int NXc = 2 * mg->m[3 * l] + 1, NYc = 2 * mg->m[3 * l + 1] + 1, NZc = 2 * mg->m[3 * l + 2] + 1;
+    const uint8_t *cmask = c->mask;
+    hipLaunchKernelGGL(k_dirichlet_unit_rows, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, (int64_t)nc, cmask, mg->ainv,
+                       np);
+    // Coloured probing: the coarse operator couples a node only to nodes within PR nodes per axis,
+    // so the non-Dirichlet dofs of one component whose node indices agree modulo 2 PR + 1 on every
+    // axis have disjoint columns: one probe vector per colour (3 (2 PR + 1)^3 probes instead of
+    // one per dof) and k_probe_scatter puts each entry into the row of the probed dof it belongs to.
+    //   Galerkin R A_f P: P spreads a coarse node over fine nodes 2I-1..2I+1, A_f reaches two fine
+    //   nodes further, R gathers fine nodes 2J-1..2J+1, so |I - J| <= 1 (PR = 1, 81 probes).
+    //   Rediscretised Q2 operator: element vertices reach two nodes away (PR = 2, 375 probes).
+    const bool gal = mg->opt.galerkin_coarse && mg->nl >= 2;
+    const int PR = gal ? 1 : 2, PP = 2 * PR + 1;
+    const int lf = l - 1;
+    const int64_t nf = gal ? mg->n[lf] : 0;
+    double *vf = nullptr, *wf = nullptr;
+    if (gal) {
       MGRC(mg_alloc(mg, (void **)&vf, sizeof(double) * nf));
       MGRC(mg_alloc(mg, (void **)&wf, sizeof(double) * nf));
-      const uint8_t *fmask = mg->ctx[lf]->mask;
-      for (int j = 0; j < nc && rc == PTQ2_OK; j++) {
-        double *row = mg->ainv + (int64_t)j * np;
-        hipLaunchKernelGGL(k_unit, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, (int64_t)nc, (int64_t)j,
-                           hm[j] ? row : e);
-        if (hm[j]) continue;
-        hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, nf, vf, 0.0);
-        hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, NXc, NYc, NXf, NYf, NZf,
-                           (const double *)e, fmask, vf);
-        rc = level_matmult<double>(mg, lf, vf, wf);
-        hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, NXf, NYf,
-                           NZf, (const double *)wf, (const double *)nullptr, (const uint8_t *)c->mask, row);
-      }
-      (void)hipFree(vf); (void)hipFree(wf);
-    } else {
-      for (int j = 0; j < nc && rc == PTQ2_OK; j++) {
-        hipLaunchKernelGGL(k_unit, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, (int64_t)nc, (int64_t)j, e);
-        rc = ptq2_a11_apply(c, e, mg->ainv + (int64_t)j * np, PTQ2_DEVICE | PTQ2_MATMULT);
-      }
     }
+    int probes = 0;
+    for (int cz = 0; cz < PP && rc == PTQ2_OK; cz++)
+      for (int cy = 0; cy < PP && rc == PTQ2_OK; cy++)
+        for (int cx = 0; cx < PP && rc == PTQ2_OK; cx++)
+          for (int comp = 0; comp < 3 && rc == PTQ2_OK; comp++) {
+            hipLaunchKernelGGL(k_probe_vec, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, PP, cx, cy, cz,
+                               comp, cmask, e);
+            if (gal) {
+              // Pi R A_f P Pi e (Pi zeroes coarse Dirichlet dofs: the probe holds none, the
+              // restriction writes zeros there)
+              const int *mf = &mg->m[3 * lf];
+              const int NXf = 2 * mf[0] + 1, NYf = 2 * mf[1] + 1, NZf = 2 * mf[2] + 1;
+              hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, nf, vf, 0.0);
+              hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, NXc, NYc, NXf, NYf,
+                                 NZf, (const double *)e, (const uint8_t *)mg->ctx[lf]->mask, vf);
+              rc = level_matmult<double>(mg, lf, vf, wf);
+              hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, NXf,
+                                 NYf, NZf, (const double *)wf, (const double *)nullptr, cmask, w);
+            } else {
+              rc = ptq2_a11_apply(c, e, w, PTQ2_DEVICE | PTQ2_MATMULT);
+            }
+            hipLaunchKernelGGL(k_probe_scatter, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, PP, cx, cy,
+                               cz, comp, cmask, (const double *)w, mg->ainv, np);
+            // bound the work queued ahead of the host
+            if (++probes % 32 == 0 && hipStreamSynchronize(mg->stream) != hipSuccess) rc = PTQ2_ERR_DEVICE;
+          }
+    if (vf) (void)hipFree(vf);
+    if (wf) (void)hipFree(wf);
+    (void)hipFree(w);
     (void)hipFree(e);
     MGRC(rc);
     hipLaunchKernelGGL(k_symmetrise, dim3((unsigned)((nn + TPB - 1) / TPB)), dim3(TPB), 0, mg->stream, np, mg->ainv);
@@ -2005,14 +2068,27 @@ int ptq2_mg_set_stream(ptq2_mg *mg, void *s) {
   return PTQ2_OK;
 }
 
+namespace {
+// the hierarchy holds device pointers of the fine context's mesh (in its graph) and coarse
+// data derived from its viscosity: both must be the ones it was built from
+int check_fine(ptq2_mg *mg) {
+  if (mg->ctx[0]->gen != mg->fine_gen)
+    return mgfail(mg, PTQ2_ERR_NOT_SETUP,
+                  "the fine context's mesh or viscosity changed after ptq2_mg_create: rebuild the hierarchy");
+  return PTQ2_OK;
+}
+}  // namespace
+
 int ptq2_mg_vcycle(ptq2_mg *mg, const double *r, double *z) {
   if (!mg || !r || !z) return PTQ2_ERR_INVALID_ARG;
+  MGRC(check_fine(mg));
   return vcycle_scaled(mg, r, 1.0, z);
 }
 
 int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_it, int restart, int *iters,
                   double *final_rnorm, double *rnorm0) {
   if (!mg || !b || !x || restart < 1 || restart > MAXRESTART || max_it < 1) return PTQ2_ERR_INVALID_ARG;
+  MGRC(check_fine(mg));
   const int64_t n = mg->n[0];
   ptq2_ctx *fc = mg->ctx[0];
   // workspace
@@ -2050,6 +2126,8 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
   double bnorm = 0.0;
   MGRC(norm2(mg, b, n, &bnorm));
   if (rnorm0) *rnorm0 = bnorm;
+  // device vectors are not scanned element by element: a NaN or Inf anywhere in b shows in its norm
+  if (!std::isfinite(bnorm)) return mgfail(mg, PTQ2_ERR_NONFINITE, "non-finite value in the right-hand side");
   int it = 0;
   double rn = bnorm;
   mg->hist.push_back(rn);
@@ -2215,6 +2293,8 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
   if (final_rnorm) *final_rnorm = tr;
   return PTQ2_OK;
 }
+
+const char *ptq2_mg_last_error(const ptq2_mg *mg) { return mg ? mg->last.c_str() : ""; }
 
 int ptq2_mg_history(const ptq2_mg *mg, double *hist, int cap, int *n) {
   if (!mg || !n) return PTQ2_ERR_INVALID_ARG;

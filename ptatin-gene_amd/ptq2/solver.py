@@ -37,6 +37,7 @@ _SOLVER_SIGS = {
                                      ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double),
                                      ctypes.POINTER(ctypes.c_double)]),
     "ptq2_mg_history": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
+    "ptq2_mg_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
 }
 _bound = False
 
@@ -51,6 +52,12 @@ def _sl():
             f.argtypes = args
         _bound = True
     return L
+
+
+def _mgcheck(code, where, h):
+    """raise PTQ2Error with the hierarchy's own message for a nonzero status"""
+    if code != _lib.OK:
+        raise _lib.PTQ2Error(code, where, _sl().ptq2_mg_last_error(h).decode())
 
 
 def default_opts():
@@ -127,7 +134,7 @@ class ViscousSolver:
         if z is None:
             z = torch.empty_like(r)
         self._stream(r)
-        check(_sl().ptq2_mg_vcycle(self._h, r.data_ptr(), z.data_ptr()), "ptq2_mg_vcycle")
+        _mgcheck(_sl().ptq2_mg_vcycle(self._h, r.data_ptr(), z.data_ptr()), "ptq2_mg_vcycle", self._h)
         return z
 
     def solve(self, b, x=None, rtol=1e-8, max_it=200, restart=30):
@@ -139,8 +146,8 @@ class ViscousSolver:
         it = ctypes.c_int()
         fr = ctypes.c_double()
         r0 = ctypes.c_double()
-        check(_sl().ptq2_mg_solve(self._h, b.data_ptr(), x.data_ptr(), rtol, max_it, restart, ctypes.byref(it),
-                                  ctypes.byref(fr), ctypes.byref(r0)), "ptq2_mg_solve")
+        _mgcheck(_sl().ptq2_mg_solve(self._h, b.data_ptr(), x.data_ptr(), rtol, max_it, restart, ctypes.byref(it),
+                                     ctypes.byref(fr), ctypes.byref(r0)), "ptq2_mg_solve", self._h)
         return x, {"iterations": it.value, "final_rnorm": fr.value, "rnorm0": r0.value,
                    "rel": fr.value / r0.value if r0.value else 0.0}
 

@@ -17,23 +17,29 @@ from ptq2 import mesh as M
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_functions():
-    txt = open(os.path.join(REPO, "include", "ptq2.h")).read()
+def header_functions(header="ptq2.h"):
+    txt = open(os.path.join(REPO, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(ptq2_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_header_symbol():
+    from ptq2 import solver as Sv
+
     L = _lib.lib()
-    names = header_functions()
-    assert len(names) >= 15
-    for n in names:
-        assert hasattr(L, n), n
-    # the ctypes table covers exactly the header
-    assert sorted(_lib.SIGNATURES) == names
+    headers = sorted(f for f in os.listdir(os.path.join(REPO, "include")) if f.endswith(".h"))
+    assert headers == ["ptq2.h", "ptq2_solver.h"]
+    names = header_functions("ptq2.h")
+    snames = header_functions("ptq2_solver.h")
+    assert len(names) >= 15 and len(snames) >= 10
     out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r" T (ptq2_\w+)", out))
-    assert set(names) <= exported
+    for n in names + snames:
+        assert hasattr(L, n), n
+        assert n in exported, n
+    # the ctypes tables cover exactly the headers
+    assert sorted(_lib.SIGNATURES) == names
+    assert sorted(Sv._SOLVER_SIGS) == snames
 
 
 def test_strerror_and_null_arguments():

@@ -253,3 +253,96 @@ def test_matmult_bc_semantics(tiny):
     assert np.all(y[~keep] == x[~keep])
     d = O.matgetdiag_a11(mx, my, mz, c, eta, mask)
     assert np.all(d[~keep] == 1.0)
+
+
+@pytest.mark.parametrize("m", [3, 6])
+def test_avx_baseline_matches_oracle(m):
+    """The CPU baseline (the AVX NEV=4 restatement of MFStokesWrapper_A11_AVX, colour-ordered
+    scatter) equals the oracle's A11 to roundoff on a jittered variable-viscosity mesh whose
+    element counts leave partial batches; its result does not depend on the thread count."""
+    P = M.config_sinker(m, n_incl=3, contrast=1e3)
+    u = M.uniform(2, P.nvel, -1, 1)
+    ref = O.a11_tensor(m, m, m, P.coords, P.eta, u)
+    y1 = O.a11_avx(m, m, m, P.coords, P.eta, u, threads=1)
+    y3 = O.a11_avx(m, m, m, P.coords, P.eta, u, threads=3)
+    assert np.linalg.norm(y1 - ref) <= 1e-13 * np.linalg.norm(ref)
+    assert np.array_equal(y1, y3)
+    assert O.avx_lib().ptq2a_flops_per_element() == 17253.0
+
+
+def _jittered(m, seed=3, shear=True):
+    c = M.jitter(m, m, m, M.q2_uniform(m, m, m), 0.1, 1.0 / (2 * m), seed)
+    return M.affine(c, M.SHEAR, [0.2, 0.0, -0.1]) if shear else c
+
+
+def test_residual_without_forces_is_the_block_matmult():
+    """FormFunction_Stokes with zero body forces and zero Dirichlet values equals
+    MatMult_MFStokes_A of (u, p) (the goldens-pinned block, tensor form) to roundoff: the
+    pressure and continuity signs of the per-point residual restatement follow the block's."""
+    m = 4
+    c = _jittered(m)
+    eta = 10.0 ** M.uniform(5, 27 * m ** 3, -1, 1)
+    mask = O.bc_mask(m, m, m, 0)
+    u = M.uniform(6, 3 * M.nnodes(m, m, m), -1, 1)
+    p = M.uniform(7, 4 * m ** 3, -1, 1)
+    fu = np.zeros((m ** 3, 27, 3))
+    Fu, Fp = O.residual(m, m, m, c, eta, mask, u, p, fu)
+    yu, yp = O.matmult_a(m, m, m, c, eta, mask, u, p)
+    assert np.abs(Fu - yu).max() <= 1e-13 * np.abs(yu).max()
+    assert np.abs(Fp - yp).max() <= 1e-13 * np.abs(yp).max()
+    assert np.array_equal(Fu[mask > 0], u[mask > 0])
+
+
+def test_residual_dirichlet_values_and_body_forces():
+    """Dirichlet values phi enter the element sums (u_D := phi_D) and the Dirichlet rows read
+    u_D - phi_D; the momentum load enters as -Be (the oracle's body force, itself checked
+    against a point-by-point assembly); a constant continuity load fp = 1 gives Fp = -int N^p:
+    minus the element volume in the constant mode (total -1 on the unit cube), and zero in the
+    linear modes of an undistorted uniform mesh (they are odd about the element centre)."""
+    m = 3
+    c = _jittered(m, shear=False)
+    E = m ** 3
+    eta = 10.0 ** M.uniform(8, 27 * E, -1, 1)
+    mask = O.bc_mask(m, m, m, 1)
+    n = 3 * M.nnodes(m, m, m)
+    u = M.uniform(9, n, -1, 1)
+    p = M.uniform(10, 4 * E, -1, 1)
+    phi = M.uniform(11, n, -1, 1)
+    fu = M.uniform(12, 81 * E, -1, 1).reshape(E, 27, 3)
+    Fu, Fp = O.residual(m, m, m, c, eta, mask, u, p, fu, phi=phi)
+    ul = np.where(mask > 0, phi, u)
+    yu, yp = O.stokes_a(m, m, m, c, eta, ul, p)
+    bf = O.body_force(m, m, m, c, fu)
+    free = mask == 0
+    assert np.abs(Fu[free] - (yu - bf)[free]).max() <= 1e-13 * np.abs(yu).max()
+    assert np.array_equal(Fu[mask > 0], u[mask > 0] - phi[mask > 0])
+    assert np.abs(Fp - yp).max() <= 1e-13 * np.abs(yp).max()
+    assert np.abs(bf - M.body_force(m, m, m, c, fu)).max() <= 1e-14 * np.abs(bf).max()
+    # continuity load alone
+    zero = np.zeros(n)
+    _, Fp1 = O.residual(m, m, m, O.mesh_uniform(m, m, m), eta, None, zero, np.zeros(4 * E), np.zeros((E, 27, 3)),
+                        fp=np.ones((E, 27)))
+    Fp1 = Fp1.reshape(E, 4)
+    assert np.allclose(Fp1[:, 0], -1.0 / E, rtol=1e-13)
+    assert np.abs(Fp1[:, 1:]).max() <= 1e-15
+    _, Fp2 = O.residual(m, m, m, c, eta, None, zero, np.zeros(4 * E), np.zeros((E, 27, 3)), fp=np.ones((E, 27)))
+    assert Fp2.reshape(E, 4)[:, 0].sum() == pytest.approx(-1.0, rel=1e-13)
+
+
+def test_residual_affine_patch():
+    """Patch test: on an affine mesh with constant viscosity, a linear velocity with constant
+    strain and the constant pressure that makes the stress tensor constant leave every interior
+    momentum row at zero (each interior node's test function integrates its gradient to 0)."""
+    m = 3
+    c = M.affine(O.mesh_uniform(m, m, m), M.SHEAR, [0.1, 0.2, 0.3])
+    E = m ** 3
+    eta = np.full(27 * E, 2.5)
+    X = c.reshape(-1, 3)
+    G = np.array([[0.3, -0.2, 0.5], [0.1, 0.4, -0.3], [0.2, 0.6, -0.7]])
+    u = (X @ G.T).reshape(-1)
+    p = np.zeros(4 * E)
+    p[0::4] = 1.7
+    Fu, _ = O.residual(m, m, m, c, eta, None, u, p, np.zeros((E, 27, 3)))
+    n1 = 2 * m + 1
+    F = Fu.reshape(n1, n1, n1, 3)[1:-1, 1:-1, 1:-1]
+    assert np.abs(F).max() <= 1e-13 * np.abs(Fu).max()
