@@ -53,6 +53,9 @@ def lib():
             "ptq2o_a11_element_matrix": [_D, _D, _D],
             "ptq2o_residual": [_i, _i, _i, _D, _D, _U8, ctypes.c_void_p, _D, _D, _D, ctypes.c_void_p, _D, _D],
             "ptq2o_body_force": [_i, _i, _i, _D, _D, _D],
+            "ptq2o_marker_lattice": [_i, _i, _i, _D, _i, _i, _i, _D, _I, _D],
+            "ptq2o_project_q1": [_i, _i, _i, ctypes.c_int64, _D, _I, _D, ctypes.c_void_p, _D,
+                                 ctypes.c_void_p],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -202,6 +205,43 @@ def body_force(mx, my, mz, coords, fu):
     y = np.zeros(3 * nnodes(mx, my, mz))
     lib().ptq2o_body_force(mx, my, mz, _f64(coords), _f64(fu), y)
     return y
+
+
+def marker_lattice(mx, my, mz, coords, n=(2, 2, 2)):
+    """Zero-perturbation lattice layout of material points (SwarmMPntStd_CoordAssignment_LatticeLayout3d
+    followed by MaterialPointCoordinateSetUp): returns (xi [np,3], wil [np], xp [np,3])."""
+    npts = mx * my * mz * n[0] * n[1] * n[2]
+    xi = np.zeros(3 * npts)
+    xp = np.zeros(3 * npts)
+    wil = np.zeros(npts, dtype=np.int32)
+    lib().ptq2o_marker_lattice(mx, my, mz, _f64(coords), n[0], n[1], n[2], xi, wil, xp)
+    return xi.reshape(-1, 3), wil, xp.reshape(-1, 3)
+
+
+def sinker_point_properties(xp, eta0=1.0, eta1=10.0, rho0=0.0, rho1=1.0, origin=(0.5, 0.5, 0.5),
+                            length=(0.5, 0.5, 0.5)):
+    """ViscousSinker_ApplyInitialMaterialGeometry_SingleInclusion (src/models/viscous_sinker/
+    model_viscous_sinker_ops.c:760-838), sphere branch: a point is in the inclusion when
+    sum_d ((x_d - o_d) / (0.5 l_d))^2 < 1; eta = eta1 / eta0, rho = -rho1 / -rho0."""
+    r = ((np.asarray(xp) - np.asarray(origin)) / (0.5 * np.asarray(length))) ** 2
+    inside = r.sum(axis=1) < 1.0
+    eta = np.where(inside, eta1, eta0).astype(np.float64)
+    rho = np.where(inside, -rho1, -rho0).astype(np.float64)
+    return eta, rho
+
+
+def project_q1(mx, my, mz, xi, wil, eta_p, rho_p=None):
+    """Q1 projection of point properties onto the quadrature points
+    (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes): returns eta_qp (and rho_qp)."""
+    xi = _f64(np.asarray(xi).reshape(-1))
+    wil = np.ascontiguousarray(wil, dtype=np.int32)
+    n = wil.size
+    eta_qp = np.zeros(27 * mx * my * mz)
+    rho_qp = None if rho_p is None else np.zeros(27 * mx * my * mz)
+    rp = None if rho_p is None else _f64(rho_p)
+    lib().ptq2o_project_q1(mx, my, mz, n, xi, wil, _f64(eta_p), None if rp is None else rp.ctypes.data, eta_qp,
+                           None if rho_qp is None else rho_qp.ctypes.data)
+    return eta_qp if rho_p is None else (eta_qp, rho_qp)
 
 
 def a11_element_matrix(elcoords, eta27):

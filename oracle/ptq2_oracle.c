@@ -769,3 +769,139 @@ void ptq2o_body_force(int mx, int my, int mz, const double *coords, const double
     scatter_element(mx, my, (int)e, Be, y);
   }
 }
+
+/* ---------------------------------------------------------------- markers */
+
+/* pTatin_ConstructNi_Q2_3D (src/element_type_Q2.c:47-69): N[i+3j+9k] = l_i(xi) l_j(eta) l_k(zeta) */
+static void q2_basis_point(const double xi[3], double N[27]) {
+  double b[3][3];
+  for (int d = 0; d < 3; d++) {
+    const double x = xi[d];
+    b[d][0] = 0.5 * x * (x - 1.0);
+    b[d][1] = (1.0 + x) * (1.0 - x);
+    b[d][2] = 0.5 * (1.0 + x) * x;
+  }
+  for (int k = 0; k < 3; k++)
+    for (int j = 0; j < 3; j++)
+      for (int i = 0; i < 3; i++) N[i + 3 * j + 9 * k] = b[0][i] * b[1][j] * b[2][k];
+}
+
+/* pTatinConstructNI_Q1_3D (src/material_point_utils.c:503-520): vertex order x fastest */
+static void q1_basis_point(const double xi[3], double N[8]) {
+  for (int c = 0; c < 8; c++) {
+    const double sx = (c & 1) ? 1.0 : -1.0, sy = (c & 2) ? 1.0 : -1.0, sz = (c & 4) ? 1.0 : -1.0;
+    N[c] = 0.125 * (1.0 + sx * xi[0]) * (1.0 + sy * xi[1]) * (1.0 + sz * xi[2]);
+  }
+}
+
+/* the Q2 element node of Q1 vertex c: 0, 2, 6, 8, 18, 20, 24, 26 */
+static int q1_vertex_node(int c) { return 2 * (c & 1) + 6 * ((c >> 1) & 1) + 18 * ((c >> 2) & 1); }
+
+/* SwarmMPntStd_CoordAssignment_LatticeLayout3d (src/material_point_std_utils.c:100-210) with a
+   zero perturbation: nx*ny*nz points per element, element-major, x fastest, at
+   xi_d = -1 + (2/n_d)(p_d + 1/2); wil = element; the physical coordinate is the Q2 interpolation
+   of the element's nodes at xi (MaterialPointCoordinateSetUp, src/ptatin3d.c:520-562, which
+   re-locates the points after the mesh geometry is applied). */
+void ptq2o_marker_lattice(int mx, int my, int mz, const double *coords, int nx, int ny, int nz, double *xi,
+                          int32_t *wil, double *xp) {
+  const int64_t nel = (int64_t)mx * my * mz;
+  const double dxi = 2.0 / nx, deta = 2.0 / ny, dzeta = 2.0 / nz;
+  int64_t p = 0;
+  for (int64_t e = 0; e < nel; e++) {
+    double elx[3][27];
+    gather_element(mx, my, (int)e, coords, elx);
+    for (int pk = 0; pk < nz; pk++)
+      for (int pj = 0; pj < ny; pj++)
+        for (int pi = 0; pi < nx; pi++, p++) {
+          double x[3] = {-1.0 + dxi * (pi + 0.5), -1.0 + deta * (pj + 0.5), -1.0 + dzeta * (pk + 0.5)};
+          double N[27];
+          q2_basis_point(x, N);
+          for (int d = 0; d < 3; d++) {
+            double s = 0.0;
+            for (int k = 0; k < 27; k++) s += N[k] * elx[d][k];
+            xp[3 * p + d] = s;
+            xi[3 * p + d] = x[d];
+          }
+          wil[p] = (int32_t)e;
+        }
+  }
+}
+
+/* SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes (src/material_point_utils.c:905-955)
+   through _SwarmUpdateGaussPropertiesLocalL2ProjectionQ1_MPntPStokes (:663-830), one rank:
+   1. every point adds w_i*eta_p, w_i*rho_p and w_i into the 27 nodes of its element, with w_i the
+      Q1 basis at the point for the 8 vertex nodes and 1 for the other 19 nodes (:705-740);
+   2. nodal values = sums / weight sums (VecPointwiseDivide, :760-761; a zero weight gives 0);
+   3. each quadrature point (3-point Gauss rule, truncated constants of QuadratureCreateGauss_3pnt_3D,
+      src/quadrature.c:136-167, q = i+3j+9k) takes the Q1 interpolation of the element's 8 vertex
+      values (:766-782,816-824), clipped to [min, max] of the element's 27 nodal values (:804-829).
+   rho_p / rho_qp may be NULL. */
+void ptq2o_project_q1(int mx, int my, int mz, int64_t npoints, const double *xi, const int32_t *wil,
+                      const double *eta_p, const double *rho_p, double *eta_qp, double *rho_qp) {
+  const int NX = 2 * mx + 1, NY = 2 * my + 1, NZ = 2 * mz + 1;
+  const int64_t N = (int64_t)NX * NY * NZ, nel = (int64_t)mx * my * mz;
+  double *A1 = (double *)calloc((size_t)N, sizeof(double));
+  double *A2 = (double *)calloc((size_t)N, sizeof(double));
+  double *Bw = (double *)calloc((size_t)N, sizeof(double));
+  for (int64_t p = 0; p < npoints; p++) {
+    double w[27], n1[8];
+    q1_basis_point(&xi[3 * p], n1);
+    for (int i = 0; i < 27; i++) w[i] = 1.0;
+    for (int c = 0; c < 8; c++) w[q1_vertex_node(c)] = n1[c];
+    const int e = wil[p];
+    const int ex = e % mx, ey = (e / mx) % my, ez = e / (mx * my);
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          const int64_t n = nodeid(NX, NY, 2 * ex + i, 2 * ey + j, 2 * ez + k);
+          const double wi = w[i + 3 * j + 9 * k];
+          A1[n] += wi * eta_p[p];
+          if (rho_p) A2[n] += wi * rho_p[p];
+          Bw[n] += wi;
+        }
+  }
+  for (int64_t n = 0; n < N; n++) {
+    A1[n] = Bw[n] != 0.0 ? A1[n] / Bw[n] : 0.0;
+    A2[n] = Bw[n] != 0.0 ? A2[n] / Bw[n] : 0.0;
+  }
+  const double s = 0.774596669241483, x1[3] = {-s, 0.0, s};
+  double NQ[NQP][8];
+  for (int q = 0; q < NQP; q++) {
+    double x[3] = {x1[q % 3], x1[(q / 3) % 3], x1[q / 9]};
+    q1_basis_point(x, NQ[q]);
+  }
+  for (int64_t e = 0; e < nel; e++) {
+    const int ex = (int)(e % mx), ey = (int)((e / mx) % my), ez = (int)(e / ((int64_t)mx * my));
+    double a1[27], a2[27];
+    double lo1 = 1.0e32, hi1 = -1.0e32, lo2 = 1.0e32, hi2 = -1.0e32;
+    for (int k = 0; k < 3; k++)
+      for (int j = 0; j < 3; j++)
+        for (int i = 0; i < 3; i++) {
+          const int64_t n = nodeid(NX, NY, 2 * ex + i, 2 * ey + j, 2 * ez + k);
+          const int l = i + 3 * j + 9 * k;
+          a1[l] = A1[n];
+          a2[l] = A2[n];
+          if (a1[l] < lo1) lo1 = a1[l];
+          if (a1[l] > hi1) hi1 = a1[l];
+          if (a2[l] < lo2) lo2 = a2[l];
+          if (a2[l] > hi2) hi2 = a2[l];
+        }
+    for (int q = 0; q < NQP; q++) {
+      double v1 = 0.0, v2 = 0.0;
+      for (int c = 0; c < 8; c++) {
+        v1 += NQ[q][c] * a1[q1_vertex_node(c)];
+        v2 += NQ[q][c] * a2[q1_vertex_node(c)];
+      }
+      if (v1 < lo1) v1 = lo1;
+      if (v1 > hi1) v1 = hi1;
+      if (v2 < lo2) v2 = lo2;
+      if (v2 > hi2) v2 = hi2;
+      eta_qp[NQP * e + q] = v1;
+      if (rho_qp) rho_qp[NQP * e + q] = v2;
+    }
+  }
+  (void)NZ;
+  free(A1);
+  free(A2);
+  free(Bw);
+}

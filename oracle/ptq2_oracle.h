@@ -74,6 +74,14 @@ void ptq2o_residual(int mx, int my, int mz, const double *coords, const double *
 /* y += body-force load of fu (81 per element) */
 void ptq2o_body_force(int mx, int my, int mz, const double *coords, const double *fu, double *y);
 
+/* material points: the zero-perturbation lattice layout (nx*ny*nz per element; xi 3 per point,
+   wil = element, xp = physical coordinate) and the Q1 projection of point viscosity/density onto
+   the quadrature points (rho_p / rho_qp may be NULL) */
+void ptq2o_marker_lattice(int mx, int my, int mz, const double *coords, int nx, int ny, int nz, double *xi,
+                          int32_t *wil, double *xp);
+void ptq2o_project_q1(int mx, int my, int mz, int64_t npoints, const double *xi, const int32_t *wil,
+                      const double *eta_p, const double *rho_p, double *eta_qp, double *rho_qp);
+
 #ifdef __cplusplus
 }
 #endif

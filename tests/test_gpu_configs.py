@@ -160,9 +160,19 @@ def test_c5_batch_concurrent_equals_sequential():
     for (P, b, ctx, S), (xs, is_), (xb, ib) in zip(probs, seq, out):
         assert ib["iterations"] == is_["iterations"]
         assert torch.equal(xs, xb)
-        r = np.linalg.norm(b - oracle_matmult(P, xb.cpu().numpy())) / np.linalg.norm(b)
-        say("  %d^3 oracle rel residual %.3e" % (P.mx, r))
-        assert r <= 1.05e-8
+        assert ib["rel"] <= 1e-8
+        xh = xb.cpu().numpy()
+        nb = np.linalg.norm(b)
+        ax_oracle = oracle_matmult(P, xh)
+        r = np.linalg.norm(b - ax_oracle) / nb
+        # The residual of one x is only defined to the rounding of the operator's element sums,
+        # which grows with the contrast (stiff elements contribute ~eta_in times the load).  The
+        # floor is the library's and the oracle's FP64 A x apart, measured on this x (host path:
+        # synchronous on the context's own stream).
+        floor = np.linalg.norm(ctx.a11(xh, flags=ptq2.MATMULT) - ax_oracle) / nb
+        say("  %d^3 oracle rel residual %.3e, FP64 residual floor %.2e" % (P.mx, r, floor))
+        assert floor <= 2e-9
+        assert r <= 1e-8 + floor + 1e-12
         S.destroy()
         ctx.destroy()
 

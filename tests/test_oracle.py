@@ -58,6 +58,101 @@ def test_goldens_a11_diag_isoviscous(goldens):
     assert _rel(d @ d, g["yy"]["compare_mf_diagA11"]) < 5e-9
 
 
+def _np_project_q1(mx, my, mz, xi, wil, eta_p, vertex_weight="q1", clip=True, interp="q1"):
+    """numpy restatement of the Q1 marker projection (src/material_point_utils.c:663-830) with
+    switches for plausible misreadings: vertex_weight "q2" weights every node by the Q2 basis
+    instead of (Q1 at the vertices, 1 elsewhere), "one" by 1 everywhere (a plain point average);
+    clip=False drops the element range clipping (a no-op: a Q1 interpolant stays inside the range
+    of its vertex values); interp "q2" interpolates the 27 nodal values with the Q2 basis."""
+    NX, NY = 2 * mx + 1, 2 * my + 1
+    N = NX * NY * (2 * mz + 1)
+
+    def b1(x, q2):
+        if q2:
+            return np.stack([0.5 * x * (x - 1), (1 + x) * (1 - x), 0.5 * (1 + x) * x], -1)
+        return np.stack([0.5 * (1 - x), np.zeros_like(x), 0.5 * (1 + x)], -1)
+
+    def n27(x, q2):
+        bx, by, bz = b1(x[:, 0], q2), b1(x[:, 1], q2), b1(x[:, 2], q2)
+        return np.einsum("pk,pj,pi->pkji", bz, by, bx).reshape(-1, 27)
+
+    W = n27(xi, vertex_weight == "q2")
+    if vertex_weight == "one":
+        W[:] = 1.0
+    if vertex_weight == "q1":
+        vert = np.zeros(27, bool)
+        vert[[0, 2, 6, 8, 18, 20, 24, 26]] = True
+        W[:, ~vert] = 1.0
+    e = wil.astype(np.int64)
+    ex, ey, ez = e % mx, (e // mx) % my, e // (mx * my)
+    loc = np.array([i + NX * (j + NY * k) for k in range(3) for j in range(3) for i in range(3)])
+    nodes = (2 * ex + NX * (2 * ey + NY * 2 * ez))[:, None] + loc[None, :]
+    A = np.bincount(nodes.ravel(), (W * eta_p[:, None]).ravel(), N)
+    B = np.bincount(nodes.ravel(), W.ravel(), N)
+    A = A / B
+    s = 0.774596669241483
+    g = np.array([-s, 0.0, s])
+    xq = np.stack([np.tile(g, 9), np.tile(np.repeat(g, 3), 3), np.repeat(g, 9)], -1)
+    NQ = n27(xq, interp == "q2")
+    E = mx * my * mz
+    ee = np.arange(E)
+    enodes = (2 * (ee % mx) + NX * (2 * ((ee // mx) % my) + NY * 2 * (ee // (mx * my))))[:, None] + loc[None, :]
+    a = A[enodes]
+    v = a @ NQ.T
+    if clip:
+        v = np.clip(v, a.min(1, keepdims=True), a.max(1, keepdims=True))
+    return v.ravel()
+
+
+def _sinker_eta(mx, my, mz, c):
+    xi, wil, xp = O.marker_lattice(mx, my, mz, c)
+    eta_p, _ = O.sinker_point_properties(xp, eta1=10.0)
+    return xi, wil, eta_p
+
+
+def test_goldens_a11_diag_marker_projection(goldens):
+    """The eta1 = 10 sinker goldens (A11 and diagonal y.y on 7 meshes): material points on the
+    zero-perturbation 2x2x2 lattice of each element of the bump mesh, the single-sphere rule,
+    the Q1 projection onto the quadrature points."""
+    seen = 0
+    for g in goldens:
+        if g["eta1"] != 10.0:
+            continue
+        mx, my, mz = g["mx"], g["my"], g["mz"]
+        c = golden_mesh(mx, my, mz)
+        xi, wil, eta_p = _sinker_eta(mx, my, mz, c)
+        eta = O.project_q1(mx, my, mz, xi, wil, eta_p)
+        mask = O.bc_mask(mx, my, mz, 0)
+        x = O.test_vector_u(mx, my, mz, c)
+        y = O.matmult_a11(mx, my, mz, c, eta, mask, x)
+        d = O.matgetdiag_a11(mx, my, mz, c, eta, mask)
+        assert _rel(y @ y, g["yy"]["compare_mf_A11"]) < 5e-9, g["source"]
+        assert _rel(d @ d, g["yy"]["compare_mf_diagA11"]) < 5e-9, g["source"]
+        seen += 1
+    assert seen == 11
+
+
+def test_marker_projection_restatements_and_misreadings(goldens):
+    """The C projection equals an independent numpy restatement (with and without the clipping,
+    which cannot act on a Q1 interpolant), and three plausible misreadings (Q2 weights at every
+    node, a plain point average, Q2 interpolation to the quadrature points) miss the 8x9x11
+    diagonal golden."""
+    g = [g for g in goldens if g["eta1"] == 10.0 and (g["mx"], g["my"], g["mz"]) == (8, 9, 11)][0]
+    mx, my, mz = 8, 9, 11
+    c = golden_mesh(mx, my, mz)
+    xi, wil, eta_p = _sinker_eta(mx, my, mz, c)
+    eta = O.project_q1(mx, my, mz, xi, wil, eta_p)
+    assert np.max(np.abs(_np_project_q1(mx, my, mz, xi, wil, eta_p) - eta)) < 1e-13
+    assert np.max(np.abs(_np_project_q1(mx, my, mz, xi, wil, eta_p, clip=False) - eta)) < 1e-13
+    mask = O.bc_mask(mx, my, mz, 0)
+    for kw in ({"vertex_weight": "q2"}, {"vertex_weight": "one"}, {"interp": "q2"}):
+        bad = _np_project_q1(mx, my, mz, xi, wil, eta_p, **kw)
+        d = O.matgetdiag_a11(mx, my, mz, c, bad, mask)
+        assert _rel(d @ d, g["yy"]["compare_mf_diagA11"]) > 1e-6, kw
+    # constancy: a uniform point field projects to the same constant at every quadrature point
+    assert np.allclose(O.project_q1(mx, my, mz, xi, wil, np.full(len(wil), 3.5)), 3.5, rtol=0, atol=1e-14)
+
+
 def test_golden_mesh_is_the_bump_mesh(goldens):
     """The flat mesh does NOT reproduce the goldens (guards against a silently wrong mesh)."""
     g = [g for g in goldens if g["eta1"] == 1.0][0]
