@@ -200,7 +200,8 @@ def cpu_baseline(seconds=8.0):
                       % (P.nvel, n_all, el_all, threads, n_one, el_one)}
 
 
-def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=0.05, rigid=False):
+def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=0.05, rigid=False,
+                     variants=("fp64", "fp32", "mixed")):
     """The viscous solve of the buoyancy problem (free-slip walls, 20 dense spheres with
     viscosity contrast 1e4) with GMG-preconditioned FGMRES: FP64 throughout ("fp64"), FP32
     Chebyshev smoothers ("fp32"), and FP32 smoothers with the finished Arnoldi vectors stored in
@@ -228,6 +229,8 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
                        % (m, restart, nlevels, emin_frac, ", rigid-body deflation of the inclusions" if rigid else "",
                           rtol)}
     for key, fp32, b32 in (("fp64", False, False), ("fp32", True, False), ("mixed", True, True)):
+        if key not in variants:
+            continue
         t = time.perf_counter()
         S = ViscousSolver(ctx, nlevels=nlevels, smooth_its=4, fp32=fp32, emin_frac=emin_frac, rigid_deflation=rigid,
                           fp32_basis=b32)
@@ -249,8 +252,10 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
               file=sys.stderr, flush=True)
         S.destroy()
         del x
-    out["speedup_fp32_vs_fp64"] = round(out["fp64_s"] / out["fp32_s"], 3)
-    out["speedup_mixed_vs_fp64"] = round(out["fp64_s"] / out["mixed_s"], 3)
+    if "fp64_s" in out and "fp32_s" in out:
+        out["speedup_fp32_vs_fp64"] = round(out["fp64_s"] / out["fp32_s"], 3)
+    if "fp64_s" in out and "mixed_s" in out:
+        out["speedup_mixed_vs_fp64"] = round(out["fp64_s"] / out["mixed_s"], 3)
     ctx.destroy()
     return out
 
@@ -327,6 +332,12 @@ def main():
     ap.add_argument("--solve", type=int, default=128, metavar="M",
                     help="also time the FP32- vs FP64-smoothed GMG solve on an M^3 mesh (0 = skip)")
     ap.add_argument("--solve-levels", type=int, default=5)
+    ap.add_argument("--solve-max-it", type=int, default=1000,
+                    help="iteration cap of the solves (a profiled run uses a few dozen)")
+    ap.add_argument("--solve-variants", default="fp64,fp32,mixed",
+                    help="comma-separated subset of fp64, fp32, mixed")
+    ap.add_argument("--solve-only", action="store_true",
+                    help="skip the apply timings and the CPU baseline: run the solve comparison only")
     ap.add_argument("--batch", action="store_true", help="also time 8 uneven solves, sequential vs concurrent")
     ap.add_argument("--rigid", choices=("off", "on", "both"), default="both",
                     help="rigid-body deflation of the stiff inclusions in the 128^3 solve comparison (both: "
@@ -337,7 +348,16 @@ def main():
                     help="time the full Stokes block (FP64 and FP32) on the contrast-1e6 sinker (default 1)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run only a plain 1 GiB device copy (calibrates HBM byte counters)")
+    ap.add_argument("--settle", type=float, default=2.0,
+                    help="seconds of untimed applies before the timed ones (clock ramp)")
+    ap.add_argument("--apply-only", action="store_true",
+                    help="time the FP64 apply only (no variants, comparisons or CPU baseline): the "
+                         "command of the profiler runs")
     args = ap.parse_args()
+    if args.apply_only:
+        args.no_cpu = args.no_fp32 = True
+        args.solve = args.block = 0
+        args.batch = False
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X; --gpus %d is not supported"
               % args.gpus, file=sys.stderr)
@@ -358,6 +378,16 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": src.numel() * 8, "copy_probe_bytes_written": dst.numel() * 8}))
         return
 
+    if args.solve_only:
+        rigid = {"off": [False], "on": [True], "both": [False, True]}[args.rigid]
+        extra = {}
+        for rg in rigid:
+            key = "solve_rigid" if rg and args.rigid == "both" else "solve"
+            extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rg, max_it=args.solve_max_it,
+                                          variants=tuple(args.solve_variants.split(",")))
+        print(json.dumps({"solve_only": True, "extra": extra}), flush=True)
+        return
+
     m = args.size
     t_setup = time.perf_counter()
     P = build_problem(m)
@@ -370,16 +400,18 @@ def main():
     setup_s = time.perf_counter() - t_setup
     print("setup %.1f s (%d elements, %d dofs)" % (setup_s, P.nel, P.nvel), file=sys.stderr, flush=True)
 
-    settle_clocks(lambda: A.ctx.a11(u, y, 0))
+    if args.settle > 0:
+        settle_clocks(lambda: A.ctx.a11(u, y, 0), args.settle)
     ms = time_applies(lambda: A.ctx.a11(u, y, 0), args.steps, args.warmup, stream)
     gdofs = P.nvel / (ms * 1e-3) / 1e9
     bytes_alg = algorithmic_bytes_a11(m)
     achieved = bytes_alg / (ms * 1e-3) / 1e9
     launches = 2
     extra = {}
-    ms_sg = time_applies(lambda: A.ctx.a11(u, y, _lib.STOREDGEO), args.steps, args.warmup, stream)
-    extra["fp64_storedgeo_ms"] = round(ms_sg, 4)
-    extra["fp64_storedgeo_gdofs"] = round(P.nvel / (ms_sg * 1e-3) / 1e9, 3)
+    if not args.apply_only:
+        ms_sg = time_applies(lambda: A.ctx.a11(u, y, _lib.STOREDGEO), args.steps, args.warmup, stream)
+        extra["fp64_storedgeo_ms"] = round(ms_sg, 4)
+        extra["fp64_storedgeo_gdofs"] = round(P.nvel / (ms_sg * 1e-3) / 1e9, 3)
     if not args.no_fp32:
         u32 = u.float()
         y32 = torch.empty_like(u32)
@@ -390,7 +422,8 @@ def main():
         torch.cuda.synchronize()
         extra["fp32_rel_err_vs_fp64"] = float(torch.linalg.norm(y32.double() - y) / torch.linalg.norm(y))
 
-    extra["hbm_copy_gbs"] = copy_bandwidth(stream)
+    if not args.apply_only:
+        extra["hbm_copy_gbs"] = copy_bandwidth(stream)
     # release the apply set-up before the larger comparisons
     del A, u, y
     u32 = y32 = None
@@ -409,7 +442,8 @@ def main():
     for rigid in variants[args.rigid] if args.solve else []:
         key = "solve_rigid" if rigid and args.rigid == "both" else "solve"
         try:
-            extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rigid)
+            extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rigid, max_it=args.solve_max_it,
+                                          variants=tuple(args.solve_variants.split(",")))
         except Exception as exc:  # the apply line above stays valid; report the failure
             extra[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     for rigid in variants[args.batch_rigid] if args.batch else []:

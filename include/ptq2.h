@@ -116,6 +116,33 @@ int ptq2_a21_apply(ptq2_ctx *ctx, const double *u, double *yp, unsigned flags);
    geometric factors (FP64 Jacobians rounded once per viscosity state); otherwise double. */
 int ptq2_stokes_apply(ptq2_ctx *ctx, const void *u, const void *p, void *yu, void *yp, unsigned flags);
 
+/* Stokes residual of FormFunction_Stokes on one rank: FormFunctionLocal_U
+   (src/stokes_form_function.c:99-242) and FormFunctionLocal_P (:478-580).  With u_loc = u whose
+   Dirichlet dofs carry the values phi (BCListInsertLocal; phi 3N, NULL = zero values):
+     Fu = A11 u_loc + A12 p - Be(fu),    Fu_D = u_D - phi_D (BCListResidualDirichlet),
+     Fp = A21 u_loc - int N^p fp,
+   fu the Fu[3] of QPntVolCoefStokes per quadrature point (81 per element, [e][q][d]) and fp its
+   Fp (27 per element, may be NULL).  Both outputs are overwritten.  The momentum and continuity
+   parts run in one element pass (the FP64 full-block kernel) plus the body-force load.  flags:
+   PTQ2_DEVICE (stream ordered; device pointers are not scanned for NaN/Inf). */
+int ptq2_stokes_residual(ptq2_ctx *ctx, const double *u, const double *p, const double *fu, const double *fp,
+                         const double *phi, double *Fu, double *Fp, unsigned flags);
+/* material points -> quadrature points: SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes
+   (src/material_point_utils.c:905-955, with its body :663-830; -coefficient_projection_type 1).
+   npoints points, each with local coordinates xi[3p..3p+2] in its element wil[p]
+   (0 <= wil < n_elements: the xi/wil fields of MPntStd) and values eta_p[p], rho_p[p] (the
+   eta/rho of MPntPStokes; rho_p may be NULL).  Writes eta_qp and rho_qp (27 per element, q =
+   a+3b+9c; rho_qp is NULL exactly when rho_p is), as the reference writes .eta/.rho of
+   QPntVolCoefStokes.  Every node sums the points of its adjacent elements in a fixed order
+   (ascending element, then point index), so the result is bitwise reproducible; it equals the
+   reference's point-order sums to rounding.  flags: PTQ2_DEVICE (every array a device pointer).
+   Synchronous (device scratch is allocated and freed inside).  Needs a single-rank box
+   (ptq2_setup_mesh, or ptq2_setup_mesh_box without a ghost layer); a box with a ghost layer is
+   PTQ2_ERR_UNSUPPORTED.  An element index out of range is PTQ2_ERR_INVALID_ARG, a non-finite
+   coordinate or value PTQ2_ERR_NONFINITE.  Pass eta_qp to ptq2_set_viscosity to use it. */
+int ptq2_project_points_q1(ptq2_ctx *ctx, int64_t npoints, const double *xi, const int32_t *wil,
+                           const double *eta_p, const double *rho_p, double *eta_qp, double *rho_qp,
+                           unsigned flags);
 /* introspection (sizes, for callers that allocate device vectors) */
 int ptq2_sizes(const ptq2_ctx *ctx, int64_t *n_velocity_dofs, int64_t *n_pressure_dofs,
                int64_t *n_elements);

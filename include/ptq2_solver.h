@@ -55,6 +55,11 @@ typedef struct {
   int rigid_deflation;      /* 1: on, 0: off (default) */
   double rigid_min_contrast; /* default 100 (>= 1) */
   int rigid_max_inclusions;  /* default 128 (1..128) */
+  /* 1: each Chebyshev step is the apply kernel with the Jacobi scaling and the three-term
+     update fused into its store (every owned node that no neighbour's face slot adds to is
+     updated there), plus a pass over the remaining lower-face nodes; results are bitwise those
+     of 0 (default): the apply storing A x, then one update pass over all dofs */
+  int fused_smoother;
 } ptq2_mg_opts;
 
 /* PTQ2_ETA_MEAN_Q1PROJ: the reference's coefficient hierarchy

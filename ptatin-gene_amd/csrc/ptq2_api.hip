@@ -209,7 +209,7 @@ int ptq2_destroy(ptq2_ctx **pc) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   dev_free(c->coords); dev_free(c->mask); dev_free(c->dir_idx); dev_free(c->eta_bm);
   dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->facebuf);
-  dev_free(c->err); dev_free(c->su); dev_free(c->sy); dev_free(c->sp); dev_free(c->syp);
+  dev_free(c->err); dev_free(c->su); dev_free(c->sy); dev_free(c->sp); dev_free(c->syp); dev_free(c->ures);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   *pc = nullptr;
@@ -272,6 +272,11 @@ int ptq2_setup_mesh_box(ptq2_ctx *c, int mx, int my, int mz, int NX, int NY, int
   Box b;
   b.mx = mx; b.my = my; b.mz = mz; b.NX = NX; b.NY = NY; b.NZ = NZ; b.i0 = i0; b.j0 = j0; b.k0 = k0;
   b.nbx = (mx + ptq2::BE - 1) / ptq2::BE; b.nby = (my + ptq2::BE - 1) / ptq2::BE; b.nbz = (mz + ptq2::BE - 1) / ptq2::BE;
+  {
+    // byte offsets of a 9^3-node tile's values from its first node fit 32 bits (fast tile paths)
+    const int64_t sy = 3 * (int64_t)NX, sz = sy * NY;
+    b.fast32 = 8 * (3 * (int64_t)ptq2::TN + (ptq2::TN - 1) * (sy + sz)) < ((int64_t)1 << 31) ? 1 : 0;
+  }
   const int64_t nb = (int64_t)b.nbx * b.nby * b.nbz;
   if (nb > INT32_MAX / 2) return fail(c, PTQ2_ERR_SIZE, "too many bricks");
 
@@ -571,15 +576,17 @@ int ptq2_stokes_apply(ptq2_ctx *c, const void *u, const void *p, void *yu, void 
                                        (float *)dyu, (float *)dyp, add, c->stream, (float *)fb));
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, (const float *)du, (float *)dyu, 0, c->stream));
   } else {
-    // momentum A11 u + A12 p, continuity A21 u (MFStokesWrapper_A_AVX in three passes)
-    if (otf)
-      HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, (const double *)du, mask, c->eta_bm, (double *)dyu, add,
-                                      c->stream, (double *)fb));
-    else
+    if (otf) {
+      // momentum A11 u + A12 p and continuity A21 u in one element pass (MFStokesWrapper_A_AVX)
+      HIPCHK(c, ptq2::launch_stokes_fp64(c->box, c->coords, (const double *)du, (const double *)dp, mask, c->eta_bm,
+                                         (double *)dyu, (double *)dyp, nullptr, add, c->stream, (double *)fb));
+    } else {
+      // resident geometry: the A11 kernel of that form, then A12 and A21
       HIPCHK(c, ptq2::launch_a11_fp64g(c->box, (const double *)du, mask, c->geo64, c->eta_bm, (double *)dyu, add,
                                        c->stream, (double *)fb));
-    HIPCHK(c, ptq2::launch_a12(c->box, c->coords, (const double *)dp, (double *)dyu, true, c->stream));
-    HIPCHK(c, ptq2::launch_a21(c->box, c->coords, (const double *)du, mask, (double *)dyp, add, c->stream));
+      HIPCHK(c, ptq2::launch_a12(c->box, c->coords, (const double *)dp, (double *)dyu, true, c->stream));
+      HIPCHK(c, ptq2::launch_a21(c->box, c->coords, (const double *)du, mask, (double *)dyp, add, c->stream));
+    }
     if (mm)
       HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, (const double *)du, (double *)dyu, 0, c->stream));
   }
@@ -589,6 +596,71 @@ int ptq2_stokes_apply(ptq2_ctx *c, const void *u, const void *p, void *yu, void 
     int bits = 0;
     if ((rc = read_err(c, &bits))) return rc;
     if (bits & ptq2::ERRBIT_NONFINITE) return fail(c, PTQ2_ERR_NONFINITE, "non-finite value in the input vector");
+  }
+  return PTQ2_OK;
+}
+
+int ptq2_stokes_residual(ptq2_ctx *c, const double *u, const double *p, const double *fu, const double *fp,
+                         const double *phi, double *Fu, double *Fp, unsigned flags) {
+  int rc = check_ready(c, true);
+  if (rc) return rc;
+  if (!u || !p || !fu || !Fu || !Fp) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
+  if (flags & ~PTQ2_DEVICE) return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
+  if ((rc = set_device(c))) return rc;
+  const bool dev = flags & PTQ2_DEVICE;
+  const int64_t nv = c->nvel, np = c->npres, ne = c->nel;
+  const double *du = u, *dp = p, *dfu = fu, *dfp = fp, *dphi = phi;
+  double *dFu = Fu, *dFp = Fp;
+  // host path: one staging buffer holds u, p, fu, fp and phi back to back
+  double *stage = nullptr;
+  struct Free {
+    double *p;
+    ~Free() { if (p) (void)hipFree(p); }
+  } fr{nullptr};
+  if (!dev) {
+    const int64_t n = nv + np + 81 * ne + (fp ? 27 * ne : 0) + (phi ? nv : 0) + nv + np;
+    if ((rc = dev_alloc(c, (void **)&stage, sizeof(double) * (size_t)n))) return rc;
+    fr.p = stage;
+    double *o = stage;
+    auto put = [&](const double *h, int64_t k) -> double * {
+      double *d = o;
+      (void)hipMemcpyAsync(d, h, sizeof(double) * (size_t)k, hipMemcpyHostToDevice, c->stream);
+      o += k;
+      return d;
+    };
+    du = put(u, nv);
+    dp = put(p, np);
+    dfu = put(fu, 81 * ne);
+    dfp = fp ? put(fp, 27 * ne) : nullptr;
+    dphi = phi ? put(phi, nv) : nullptr;
+    dFu = o;
+    dFp = o + nv;
+    HIPCHK(c, hipGetLastError());
+    if ((rc = clear_err(c))) return rc;
+    HIPCHK(c, ptq2::launch_scan_finite<double>(o - stage, stage, c->err, c->stream));
+  }
+  // u_loc: u with the Dirichlet values inserted (BCListInsertLocal, src/dmda_bcs.c:547-573)
+  const double *ul = du;
+  if (c->ndir) {
+    if ((rc = ensure(c, (void **)&c->ures, &c->ures_bytes, sizeof(double) * (size_t)nv))) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->ures, du, sizeof(double) * (size_t)nv, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, dphi, c->ures, dphi ? 0 : 1, c->stream));
+    ul = c->ures;
+  }
+  void *fb = ptq2_internal_facebuf(c, &rc);
+  if (rc) return rc;
+  // Fu = A11 u_loc + A12 p, Fp = A21 u_loc - int N^p fp (one element pass), then Fu -= Be
+  HIPCHK(c, ptq2::launch_stokes_fp64(c->box, c->coords, ul, dp, nullptr, c->eta_bm, dFu, dFp, dfp, false, c->stream,
+                                     (double *)fb));
+  HIPCHK(c, ptq2::launch_body_force(c->box, c->coords, dfu, dFu, true, c->stream, -1.0));
+  // Dirichlet rows: Fu_D = u_D - phi_D (BCListResidualDirichlet, src/dmda_bcs.c:614-648)
+  HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, du, dFu, 3, c->stream, dphi));
+  if (!dev) {
+    HIPCHK(c, hipMemcpyAsync(Fu, dFu, sizeof(double) * (size_t)nv, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(Fp, dFp, sizeof(double) * (size_t)np, hipMemcpyDeviceToHost, c->stream));
+    int bits = 0;
+    if ((rc = read_err(c, &bits))) return rc;
+    if (bits & ptq2::ERRBIT_NONFINITE) return fail(c, PTQ2_ERR_NONFINITE, "non-finite value in an input vector");
   }
   return PTQ2_OK;
 }

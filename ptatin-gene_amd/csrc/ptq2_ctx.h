@@ -38,6 +38,9 @@ struct ptq2_ctx {
   // memory phases of some bricks overlapped with the arithmetic of others, which 8 coloured
   // launches of a few waves per SIMD do not.
   void *facebuf = nullptr;
+  // u with the Dirichlet values inserted, for the residual (allocated on first use)
+  double *ures = nullptr;
+  size_t ures_bytes = 0;
   // staging for host-pointer calls
   void *su = nullptr, *sy = nullptr;
   size_t su_bytes = 0, sy_bytes = 0;

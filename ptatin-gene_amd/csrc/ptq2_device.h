@@ -46,6 +46,7 @@ struct Box {
   int NX, NY, NZ;
   int i0, j0, k0;
   int nbx, nby, nbz;
+  int fast32;  // element offsets inside a 9^3-node tile fit 32 bits (the fast tile paths)
 };
 
 // 1D Q2 Lagrange basis on nodes {-1,0,1} at the Gauss points {-sqrt(3/5),0,sqrt(3/5)}:
@@ -160,6 +161,128 @@ __device__ __forceinline__ int64_t tile_gidx(const Tile &t, const TileLane &L, i
   return L.gcol + L.sy * min(y, t.tny - 1) + L.sz * min(z, t.tnz - 1);
 }
 
+// Fast tile traffic for a full 9x9x9 tile (every brick of a mesh whose element counts are
+// multiples of 4, and the interior bricks of any mesh).  The tile's first node gives one
+// wave-uniform base pointer per field; a lane addresses its rows with 32-bit element offsets
+// from it.  Row r = 2 it + p of the tile (y = r % 9, z = r / 9) sits at element offset
+//   sy y + sz z = sy r + (sz - 9 sy) (r / 9)          (sy = 3 NX, sz = 3 NX NY)
+// and its column at LDS offset TRS y + TPS z = 10 r + 2 (r / 9) (TPS = 9 TRS + 2).  For a
+// lane of parity p, (2 it + p) / 9 = (2 it) / 9 except on the odd lanes of the steps with
+// (2 it) % 9 == 8, whose row wraps to the next plane: those steps use a second per-lane base.
+// All per-step terms are then compile-time or wave-uniform, so a step costs one 32-bit add
+// per access instead of a 64-bit multiply chain, and the LDS offsets are instruction
+// immediates.  The last step's odd lanes (row 81) and the idle lanes 54-63 read row 0 and
+// store nothing.
+__host__ __device__ __forceinline__ bool tile_fast(const Box &b, const Tile &t) {
+  return b.fast32 && t.tnx == TN && t.tny == TN && t.tnz == TN;
+}
+struct FastLane {
+  int p;              // row parity 0/1, 2 = idle lane
+  int col;            // column inside the row (0..26)
+  int ii, c;          // node column and component
+  int lds0, ldsw;     // LDS offset of row (p,0); the wrap-step base
+  uint32_t e0, ew;    // element offset of row (p,0); the wrap-step base
+};
+__host__ __device__ __forceinline__ FastLane fast_lane(const Box &b, int lane) {
+  FastLane L;
+  L.p = lane < 2 * RLEN ? lane / RLEN : 2;
+  L.col = L.p < 2 ? lane - RLEN * L.p : 0;
+  L.ii = L.col / 3;
+  L.c = L.col - 3 * L.ii;
+  const uint32_t sy = 3u * (uint32_t)b.NX, sz = sy * (uint32_t)b.NY;
+  const int odd = L.p == 1;
+  L.lds0 = L.c * TCS + tx(L.ii) + (odd ? TRS : 0);
+  L.ldsw = L.lds0 + (odd ? 2 : 0);
+  L.e0 = (uint32_t)L.col + (odd ? sy : 0u);
+  L.ew = L.e0 + (odd ? sz - 9u * sy : 0u);
+  return L;
+}
+// uniform element offset of step it for the even row 2 it
+__host__ __device__ __forceinline__ uint32_t fast_step(const Box &b, int it) {
+  const uint32_t sy = 3u * (uint32_t)b.NX, sz = sy * (uint32_t)b.NY;
+  return 2u * (uint32_t)it * sy + (sz - 9u * sy) * (uint32_t)((2 * it) / 9);
+}
+__host__ __device__ __forceinline__ constexpr int fast_lds_step(int it) { return 20 * it + 2 * ((2 * it) / 9); }
+__host__ __device__ __forceinline__ constexpr bool fast_wrap(int it) { return (2 * it) % 9 == 8; }
+// this lane's element offset at step it (row 0 for the lanes that store nothing at that step)
+__host__ __device__ __forceinline__ uint32_t fast_eoff(const Box &b, const FastLane &L, int it) {
+  if (it == TPAIRS - 1 && L.p == 1) return (uint32_t)L.col;
+  return (fast_wrap(it) ? L.ew : L.e0) + fast_step(b, it);
+}
+// (y, z) of this lane's row at step it
+__host__ __device__ __forceinline__ void fast_row(const FastLane &L, int it, int &y, int &z) {
+  const int odd = L.p == 1, y0 = (2 * it) % TN, z0 = (2 * it) / TN;
+  y = fast_wrap(it) ? (odd ? 0 : y0) : y0 + odd;
+  z = fast_wrap(it) ? z0 + odd : z0;
+}
+__host__ __device__ __forceinline__ bool fast_stores(const FastLane &L, int it) {
+  return L.p == 0 || (L.p == 1 && it < TPAIRS - 1);
+}
+// element e of a field from a wave-uniform base: the byte offset is formed in 32 bits, so the
+// access is one global instruction with a scalar base and a 32-bit vector offset
+template <typename S>
+__device__ __forceinline__ S &fast_at(S *base, uint32_t e) {
+  const uint32_t bo = e * (uint32_t)sizeof(S);
+  return *reinterpret_cast<S *>(reinterpret_cast<char *>(base) + bo);
+}
+template <typename S>
+__device__ __forceinline__ const S &fast_at(const S *base, uint32_t e) {
+  const uint32_t bo = e * (uint32_t)sizeof(S);
+  return *reinterpret_cast<const S *>(reinterpret_cast<const char *>(base) + bo);
+}
+
+// two fields, the second masked (Dirichlet dofs read as zero) if HM
+template <bool HM, typename T, typename S>
+__device__ __forceinline__ void load_tile2_fast(const Box &b, const Tile &t, const S *__restrict__ g1,
+                                                T *__restrict__ s1, const S *__restrict__ g2,
+                                                const uint8_t *__restrict__ mask2, T *__restrict__ s2, int lane) {
+  const FastLane L = fast_lane(b, lane);
+  const S *b1 = g1 + 3 * t.node0, *b2 = g2 + 3 * t.node0;
+  const uint8_t *bm = HM ? mask2 + 3 * t.node0 : nullptr;
+  S v1[TPAIRS], v2[TPAIRS];
+  int mk[TPAIRS];
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    const uint32_t e = fast_eoff(b, L, it);
+    v1[it] = fast_at(b1, e);
+    v2[it] = fast_at(b2, e);
+    mk[it] = HM ? (int)fast_at(bm, e) : 0;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    if (fast_stores(L, it)) {
+      const int o = (fast_wrap(it) ? L.ldsw : L.lds0) + fast_lds_step(it);
+      s1[o] = static_cast<T>(v1[it]);
+      s2[o] = mk[it] ? T(0) : static_cast<T>(v2[it]);
+    }
+  }
+}
+// one field, masked if HM
+template <bool HM, typename T, typename S>
+__device__ __forceinline__ void load_tile_fast(const Box &b, const Tile &t, const S *__restrict__ g,
+                                               const uint8_t *__restrict__ mask, T *__restrict__ s, int lane) {
+  const FastLane L = fast_lane(b, lane);
+  const S *b1 = g + 3 * t.node0;
+  const uint8_t *bm = HM ? mask + 3 * t.node0 : nullptr;
+  S v[TPAIRS];
+  int mk[TPAIRS];
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    const uint32_t e = fast_eoff(b, L, it);
+    v[it] = fast_at(b1, e);
+    mk[it] = HM ? (int)fast_at(bm, e) : 0;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    if (fast_stores(L, it)) {
+      const int o = (fast_wrap(it) ? L.ldsw : L.lds0) + fast_lds_step(it);
+      s[o] = mk[it] ? T(0) : static_cast<T>(v[it]);
+    }
+  }
+}
+
 // Stage an interlaced 3-component nodal field into an LDS tile.  When mask is given,
 // Dirichlet dofs read as zero (BCListInsertLocalZero, src/dmda_bcs.c:581-608).  Tile
 // nodes outside the box read as zero.
@@ -192,6 +315,13 @@ __device__ __forceinline__ void load_tile_m(const Box &b, const Tile &t, const B
 template <typename T, typename S>
 __device__ __forceinline__ void load_tile(const Box &b, const Tile &t, const BrickId &br, const S *__restrict__ g,
                                           const uint8_t *__restrict__ mask, T *__restrict__ s, int lane) {
+  if (tile_fast(b, t)) {
+    if (mask)
+      load_tile_fast<true>(b, t, g, mask, s, lane);
+    else
+      load_tile_fast<false>(b, t, g, mask, s, lane);
+    return;
+  }
   if (mask)
     load_tile_m<true>(b, t, br, g, mask, s, lane);
   else
@@ -233,6 +363,13 @@ template <typename T, typename S>
 __device__ __forceinline__ void load_tile2(const Box &b, const Tile &t, const BrickId &br, const S *__restrict__ g1,
                                            T *__restrict__ s1, const S *__restrict__ g2,
                                            const uint8_t *__restrict__ mask2, T *__restrict__ s2, int lane) {
+  if (tile_fast(b, t)) {
+    if (mask2)
+      load_tile2_fast<true>(b, t, g1, s1, g2, mask2, s2, lane);
+    else
+      load_tile2_fast<false>(b, t, g1, s1, g2, mask2, s2, lane);
+    return;
+  }
   if (mask2)
     load_tile2_m<true>(b, t, br, g1, s1, g2, mask2, s2, lane);
   else
@@ -284,10 +421,79 @@ __device__ __forceinline__ void store_tile(const Box &b, const BrickId &br, cons
 // applies.  k_gather_faces adds the slots into their owners.
 constexpr int TILEVALS = 3 * TN * TN * TN;
 constexpr int FACEVALS = 3 * 3 * TN * TN;
-template <typename S, typename T>
+// One Chebyshev step fused into the apply's store (ChebychevRN, src/ksp_chebyrn.c:344-411):
+// out = cr dinv (b - y) + ck pkm1 + cx x (pkm1 null: cr dinv (b - y) + x) with y = (A x)_i,
+// y = x on Dirichlet dofs (MatMult semantics).  The explicit FMA order is shared by the
+// apply's store and the face pass, so a dof gets the same bits whichever of them updates it.
+template <typename T>
+struct ChebOut {
+  const T *b, *dinv, *x, *pkm1;
+  T *out;
+  const uint8_t *mask;
+  T cr, ck, cx;
+};
+template <typename T>
+__host__ __device__ __forceinline__ T cheb_update(const ChebOut<T> &co, int64_t i, T y) {
+  const T v = (co.cr * co.dinv[i]) * (co.b[i] - y);
+  return co.pkm1 ? fma(co.cx, co.x[i], fma(co.ck, co.pkm1[i], v)) : v + co.x[i];
+}
+
+// the full-tile case with the addressing of load_tile2_fast
+template <bool CH, typename S, typename T>
+__device__ __forceinline__ void store_tile_own_fast(const Box &b, const BrickId &br, const Tile &t,
+                                                    const S *__restrict__ s, T *__restrict__ g,
+                                                    T *__restrict__ face, bool add_mode, int lane,
+                                                    const ChebOut<T> *co) {
+  const FastLane L = fast_lane(b, lane);
+  const bool ownx = L.ii < 2 * BE || br.bx == b.nbx - 1;
+  const bool lasty = br.by == b.nby - 1, lastz = br.bz == b.nbz - 1;
+  T *gb = g + 3 * t.node0;
+  T *fo = face + (int64_t)br.lin * FACEVALS + L.c;
+  T v[TPAIRS];
+  if (add_mode) {
+#pragma unroll
+    for (int it = 0; it < TPAIRS; it++) v[it] = fast_at(gb, fast_eoff(b, L, it));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+#pragma unroll
+  for (int it = 0; it < TPAIRS; it++) {
+    if (!fast_stores(L, it)) continue;
+    int y, z;
+    fast_row(L, it, y, z);
+    const bool owny = y < 2 * BE || lasty, ownz = z < 2 * BE || lastz;
+    const S val = s[(fast_wrap(it) ? L.ldsw : L.lds0) + fast_lds_step(it)];
+    if (ownx && owny && ownz) {
+      const uint32_t e = fast_eoff(b, L, it);
+      if constexpr (CH) {
+        // final unless a lower neighbour's face slot still adds to it (or Dirichlet: y = x)
+        const int64_t gi = 3 * t.node0 + e;
+        const bool dir = co->mask && co->mask[gi];
+        const bool lower = (L.ii == 0 && br.bx > 0) || (y == 0 && br.by > 0) || (z == 0 && br.bz > 0);
+        if (dir || !lower)
+          co->out[gi] = cheb_update(*co, gi, dir ? co->x[gi] : T(val));
+        else
+          fast_at(gb, e) = T(val);
+      } else {
+        fast_at(gb, e) = add_mode ? T(v[it] + val) : T(val);
+      }
+    } else {
+      const int slot = !ownx ? z * TN + y : (!owny ? TN * TN + z * TN + L.ii : 2 * TN * TN + y * TN + L.ii);
+      fo[3 * slot] = T(val);
+    }
+  }
+}
+
+// CH: the Chebyshev update of co is applied to every owned node that is final here (see
+// store_tile_own_fast); the others are stored into g for the face pass.
+template <bool CH = false, typename S, typename T>
 __device__ __forceinline__ void store_tile_own(const Box &b, const BrickId &br, const Tile &t, const S *__restrict__ s,
-                                               T *__restrict__ g, T *__restrict__ face, bool add_mode, int lane) {
+                                               T *__restrict__ g, T *__restrict__ face, bool add_mode, int lane,
+                                               const ChebOut<T> *co = nullptr) {
   asm volatile("" : "+v"(lane));  // see store_tile
+  if (tile_fast(b, t)) {
+    store_tile_own_fast<CH>(b, br, t, s, g, face, add_mode, lane, co);
+    return;
+  }
   const TileLane L = tile_lane(b, t, br, lane);
   const int col = lane - RLEN * (L.p < 2 ? L.p : 0);
   const int ii = col / 3, c = col - 3 * ii;
@@ -312,7 +518,17 @@ __device__ __forceinline__ void store_tile_own(const Box &b, const BrickId &br, 
     const bool owny = y < 2 * BE || lasty, ownz = z < 2 * BE || lastz;
     const S val = s[L.lds_col + TRS * y + TPS * z];
     if (ownx && owny && ownz) {
-      g[tile_gidx(t, L, y, z)] = add_mode ? T(v[it] + val) : T(val);
+      const int64_t gi = tile_gidx(t, L, y, z);
+      if constexpr (CH) {
+        const bool dir = co->mask && co->mask[gi];
+        const bool lower = (ii == 0 && br.bx > 0) || (y == 0 && br.by > 0) || (z == 0 && br.bz > 0);
+        if (dir || !lower)
+          co->out[gi] = cheb_update(*co, gi, dir ? co->x[gi] : T(val));
+        else
+          g[gi] = T(val);
+      } else {
+        g[gi] = add_mode ? T(v[it] + val) : T(val);
+      }
     } else {
       const int slot = !ownx ? z * TN + y : (!owny ? TN * TN + z * TN + ii : 2 * TN * TN + y * TN + ii);
       fo[3 * slot] = T(val);
@@ -476,6 +692,31 @@ __device__ __forceinline__ void stress_action(const T A[3][3], const T g[3][3], 
   for (int k = 0; k < 3; k++)
 #pragma unroll
     for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * A[0][k] + S[1][l] * A[1][k] + S[2][l] * A[2][k];
+}
+
+// The same with the pressure of the full block: S -= wp I (wp = w p_q), which adds
+// -w p_q adj(J)^T, the A12 term of QuadratureAction_A_AVX (src/stokes_operators_avx.c:376-437);
+// returns trace(M) = det div u for the A21 term.
+template <typename T>
+__device__ __forceinline__ T stress_action_p(const T A[3][3], const T g[3][3], T s, T wp, T dv[3][3]) {
+  T M[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) M[k][l] = g[0][l] * A[k][0] + g[1][l] * A[k][1] + g[2][l] * A[k][2];
+  T S[3][3];
+  T s2 = s + s;
+  S[0][0] = fma(s2, M[0][0], -wp);
+  S[1][1] = fma(s2, M[1][1], -wp);
+  S[2][2] = fma(s2, M[2][2], -wp);
+  S[0][1] = S[1][0] = s * (M[0][1] + M[1][0]);
+  S[0][2] = S[2][0] = s * (M[0][2] + M[2][0]);
+  S[1][2] = S[2][1] = s * (M[1][2] + M[2][1]);
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int l = 0; l < 3; l++) dv[k][l] = S[0][l] * A[0][k] + S[1][l] * A[1][k] + S[2][l] * A[2][k];
+  return M[0][0] + M[1][1] + M[2][2];
 }
 
 // x-transpose accumulation of one quadrature point's dv into the row accumulators

@@ -11,43 +11,127 @@
 
 namespace ptq2 {
 
+// ------------------------------------------------------- pressure basis helper
+// P3D_ConstructNi_P1GRel_3D (src/element_utils_q2.c:183-224):
+// N^p = [1, (x-xbar)/Lx, (y-ybar)/Ly, (z-zbar)/Lz], xbar the mean of the 27 nodes,
+// Lx = x_14 - x_12, Ly = y_16 - y_10, Lz = z_22 - z_4.
+struct PBasis {
+  double avg[3], inv[3];
+};
+__device__ __forceinline__ PBasis pbasis_setup(const double *Xs, int base) {
+  PBasis pb;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) s += tv(Xs, base, c, i, j, k);
+    pb.avg[c] = s * (1.0 / 27.0);
+  }
+  pb.inv[0] = 1.0 / (tv(Xs, base, 0, 2, 1, 1) - tv(Xs, base, 0, 0, 1, 1));
+  pb.inv[1] = 1.0 / (tv(Xs, base, 1, 1, 2, 1) - tv(Xs, base, 1, 1, 0, 1));
+  pb.inv[2] = 1.0 / (tv(Xs, base, 2, 1, 1, 2) - tv(Xs, base, 2, 1, 1, 0));
+  return pb;
+}
+
+
 // ----------------------------------------------------------------- A11 FP64
 
 // geometry of quadrature point qa of the current row: adjugate and s = w eta / det
 template <int qa>
-__device__ __forceinline__ void geo_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
-                                       double wbc, double eta, double A[3][3], double &s) {
+__device__ __forceinline__ double geo_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                         double wbc, double eta, double A[3][3], double &s) {
   double J[3][3];
   qp_grad<qa>(xr0, xr1, xr2, J);
   double det = adjugate(J, A);
   s = (gw1(qa) * wbc * eta) * rcp64(det);
+  return det;
+}
+
+// Pressure-side state of the full block (BLK): the element's P1GRel basis set-up and
+// pressure, the continuity accumulators and, for the residual (fp != nullptr), the 27
+// quadrature values of the continuity source.
+struct BlockState {
+  PBasis pb;
+  double pe[4];
+  double Yp[4];
+  const double *fp;
+};
+
+// pressure basis values N^p_1..3 at quadrature point qa of the row (x from the B-B row data)
+template <int qa>
+__device__ __forceinline__ void pbasis_qp(const double xr0[3][3], const PBasis &pb, double np[3]) {
+  double x[3];
+  qp_value<qa>(xr0, x);
+#pragma unroll
+  for (int d = 0; d < 3; d++) np[d] = (x[d] - pb.avg[d]) * pb.inv[d];
+}
+
+// one quadrature point of the velocity phase
+template <int qa, bool BLK>
+__device__ __forceinline__ void a11_qp64(const double ur0[3][3], const double ur1[3][3], const double ur2[3][3],
+                                         const double A[3][3], double s, double wbc, const double np[3],
+                                         double wdet, int q, BlockState *bs, double P0[3][3], double P1[3][3],
+                                         double P2[3][3]) {
+  double g[3][3], dv[3][3];
+  qp_grad<qa>(ur0, ur1, ur2, g);
+  if constexpr (BLK) {
+    const double w = gw1(qa) * wbc;
+    const double pq = bs->pe[0] + bs->pe[1] * np[0] + bs->pe[2] * np[1] + bs->pe[3] * np[2];
+    const double f = -w * stress_action_p(A, g, s, w * pq, dv);
+    double fs = 0.0;
+    if (bs->fp) fs = -wdet * bs->fp[q];
+    bs->Yp[0] += f + fs;
+#pragma unroll
+    for (int d = 0; d < 3; d++) bs->Yp[1 + d] += (f + fs) * np[d];
+  } else {
+    stress_action(A, g, s, dv);
+  }
+  acc_row<qa>(dv, P0, P1, P2);
 }
 
 // one quadrature row (runtime qb,qc) of the FP64 A11 element: geometry of its three
 // points from the coordinate tile, velocity gradients from the velocity tile, the
 // stress action and the x-transpose into P, then the y- and z-transposes into Y.
 // On entry ca holds column 0 of the coordinate tile; on exit it holds it again (for
-// the next row), its reads in flight behind the stress work.
+// the next row), its reads in flight behind the stress work.  BLK adds the pressure and
+// continuity terms of the full block.
+template <bool BLK>
 __device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, int base, const Coef1 &cb,
                                           const Coef1 &cc, double wbc, const double e3[3], double ca[9],
-                                          double cb9[9], double Y[3][3][3][3]) {
+                                          double cb9[9], double Y[3][3][3][3], int row, BlockState *bs) {
   double A0[3][3], A1[3][3], A2[3][3], s0, s1, s2;
+  double np[3][3], wd[3] = {0.0, 0.0, 0.0};
   {
     double xr0[3][3], xr1[3][3], xr2[3][3];
     row_data_pipe(Xs, Us, base, cb, cc, ca, cb9, xr0, xr1, xr2);
-    geo_qp<0>(xr0, xr1, xr2, wbc, e3[0], A0, s0);
-    geo_qp<1>(xr0, xr1, xr2, wbc, e3[1], A1, s1);
-    geo_qp<2>(xr0, xr1, xr2, wbc, e3[2], A2, s2);
+    const double d0 = geo_qp<0>(xr0, xr1, xr2, wbc, e3[0], A0, s0);
+    const double d1 = geo_qp<1>(xr0, xr1, xr2, wbc, e3[1], A1, s1);
+    const double d2 = geo_qp<2>(xr0, xr1, xr2, wbc, e3[2], A2, s2);
+    if constexpr (BLK) {
+      pbasis_qp<0>(xr0, bs->pb, np[0]);
+      pbasis_qp<1>(xr0, bs->pb, np[1]);
+      pbasis_qp<2>(xr0, bs->pb, np[2]);
+      if (bs->fp) {
+        wd[0] = gw1(0) * wbc * d0;
+        wd[1] = gw1(1) * wbc * d1;
+        wd[2] = gw1(2) * wbc * d2;
+      }
+    } else {
+      (void)d0; (void)d1; (void)d2;
+    }
   }
   row_fence();
   double ur0[3][3], ur1[3][3], ur2[3][3];
   row_data_pipe(Us, Xs, base, cb, cc, cb9, ca, ur0, ur1, ur2);
   double P0[3][3], P1[3][3], P2[3][3];
   zero33(P0); zero33(P1); zero33(P2);
-  double g[3][3], dv[3][3];
-  qp_grad<0>(ur0, ur1, ur2, g); stress_action(A0, g, s0, dv); acc_row<0>(dv, P0, P1, P2);
-  qp_grad<1>(ur0, ur1, ur2, g); stress_action(A1, g, s1, dv); acc_row<1>(dv, P0, P1, P2);
-  qp_grad<2>(ur0, ur1, ur2, g); stress_action(A2, g, s2, dv); acc_row<2>(dv, P0, P1, P2);
+  a11_qp64<0, BLK>(ur0, ur1, ur2, A0, s0, wbc, np[0], wd[0], 3 * row, bs, P0, P1, P2);
+  a11_qp64<1, BLK>(ur0, ur1, ur2, A1, s1, wbc, np[1], wd[1], 3 * row + 1, bs, P0, P1, P2);
+  a11_qp64<2, BLK>(ur0, ur1, ur2, A2, s2, wbc, np[2], wd[2], 3 * row + 2, bs, P0, P1, P2);
   row_fence();
   acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
   row_fence();
@@ -76,10 +160,18 @@ __device__ __forceinline__ void load3(const double *__restrict__ ep, int row, do
   for (int q = 0; q < 3; q++) e3[q] = ep[(3 * row + q) * BEL];
 }
 
-__global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double *__restrict__ coords,
-                                                  const double *__restrict__ u, const uint8_t *__restrict__ mask,
-                                                  const double *__restrict__ eta_bm, double *__restrict__ y,
-                                                  int add_mode, double *__restrict__ facebuf) {
+// BLK = false: the A11 apply.  BLK = true: the full block [A11 A12; A21 0] in the same single
+// element pass (MFStokesWrapper_A_AVX, src/stokes_operators_avx.c:439-636): y gets A11 u + A12 p,
+// yp gets A21 u (and, with fp, minus the continuity source: FormFunctionLocal_P's
+// -int N^p fp, src/stokes_form_function.c:478-580).
+// CH: store with the fused Chebyshev update of co (store_tile_own<true>).
+template <bool BLK, bool CH>
+__global__ __launch_bounds__(BEL) void k_a11_fp64_t(Box b, int color, const double *__restrict__ coords,
+                                                    const double *__restrict__ u, const uint8_t *__restrict__ mask,
+                                                    const double *__restrict__ eta_bm, double *__restrict__ y,
+                                                    int add_mode, double *__restrict__ facebuf,
+                                                    const double *__restrict__ p, double *__restrict__ yp,
+                                                    const double *__restrict__ fp, ChebOut<double> co) {
   __shared__ double Xs[3 * TCS];
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
@@ -101,6 +193,20 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
       for (int j = 0; j < 3; j++)
 #pragma unroll
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
+  BlockState bs;
+  int64_t e = 0;
+  if constexpr (BLK) {
+    bs.Yp[0] = bs.Yp[1] = bs.Yp[2] = bs.Yp[3] = 0.0;
+    bs.pe[0] = bs.pe[1] = bs.pe[2] = bs.pe[3] = 0.0;
+    bs.fp = nullptr;
+    if (active) {
+      e = lane_element(b, t, lane);
+      bs.pb = pbasis_setup(Xs, base);
+#pragma unroll
+      for (int i = 0; i < 4; i++) bs.pe[i] = p[4 * e + i];
+      if (fp) bs.fp = fp + 27 * e;
+    }
+  }
   if (active) {
     double ca[9], cb9[9];
     ld_col(Xs, base, 0, ca);
@@ -109,7 +215,12 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
       const int qb = row % 3, qc = row / 3;
       double e3[3] = {en[0], en[1], en[2]};
       if (row < 8) load3(ep, row + 1, en);
-      a11_row64(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y);
+      a11_row64<BLK>(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y, row, &bs);
+    }
+    if constexpr (BLK) {
+      double *o = yp + 4 * e;
+#pragma unroll
+      for (int i = 0; i < 4; i++) o[i] = add_mode ? o[i] + bs.Yp[i] : bs.Yp[i];
     }
   }
   __syncthreads();
@@ -119,6 +230,8 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64(Box b, int color, const double
   __syncthreads();
   if (color >= 0)
     store_tile(b, br, t, Xs, y, add_mode != 0, lane);
+  else if constexpr (CH)
+    store_tile_own<true>(b, br, t, Xs, y, facebuf, false, lane, &co);
   else
     store_tile_own(b, br, t, Xs, y, facebuf, add_mode != 0, lane);
 }
@@ -179,9 +292,11 @@ __device__ __forceinline__ void slab_z32(const float *Us, int base, const Coef1 
   }
 }
 
+template <bool CH>
 __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
                                                   const uint8_t *__restrict__ mask, const float *__restrict__ fac,
-                                                  float *__restrict__ y, int add_mode, float *__restrict__ facebuf) {
+                                                  float *__restrict__ y, int add_mode, float *__restrict__ facebuf,
+                                                  ChebOut<float> co) {
   // LDS: the FP32 velocity tile and one quadrature row of factors ([qa*9+m][lane]) during
   // the rows, then the output tile in FP64 (LDS float atomics are many times slower than
   // ds_add_f64 on gfx950, and the wider sum costs nothing in the result)
@@ -254,6 +369,8 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   __syncthreads();
   if (color >= 0)
     store_tile(b, br, t, Ts, y, add_mode != 0, lane);
+  else if constexpr (CH)
+    store_tile_own<true>(b, br, t, Ts, y, facebuf, false, lane, &co);
   else
     store_tile_own(b, br, t, Ts, y, facebuf, add_mode != 0, lane);
 }
@@ -527,32 +644,6 @@ __global__ __launch_bounds__(BEL) void k_stokes_fp32(Box b, int color, const flo
     store_tile_own(b, br, t, Ts, yu, facebuf, add_mode != 0, lane);
 }
 
-// ------------------------------------------------------- pressure basis helper
-// P3D_ConstructNi_P1GRel_3D (src/element_utils_q2.c:183-224):
-// N^p = [1, (x-xbar)/Lx, (y-ybar)/Ly, (z-zbar)/Lz], xbar the mean of the 27 nodes,
-// Lx = x_14 - x_12, Ly = y_16 - y_10, Lz = z_22 - z_4.
-struct PBasis {
-  double avg[3], inv[3];
-};
-__device__ __forceinline__ PBasis pbasis_setup(const double *Xs, int base) {
-  PBasis pb;
-#pragma unroll
-  for (int c = 0; c < 3; c++) {
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < 3; k++)
-#pragma unroll
-      for (int j = 0; j < 3; j++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) s += tv(Xs, base, c, i, j, k);
-    pb.avg[c] = s * (1.0 / 27.0);
-  }
-  pb.inv[0] = 1.0 / (tv(Xs, base, 0, 2, 1, 1) - tv(Xs, base, 0, 0, 1, 1));
-  pb.inv[1] = 1.0 / (tv(Xs, base, 1, 1, 2, 1) - tv(Xs, base, 1, 1, 0, 1));
-  pb.inv[2] = 1.0 / (tv(Xs, base, 2, 1, 1, 2) - tv(Xs, base, 2, 1, 1, 0));
-  return pb;
-}
-
 // FP64 geometry -> FP32 factors, with validation of det > 0 and eta > 0.
 //   fac  [brick][q][9][lane]: F = sqrt(w eta / det) adj(J)
 //   facb [brick][q][4][lane] (optional, for the FP32 full block): sqrt(w det / eta) (so that
@@ -821,29 +912,30 @@ __global__ __launch_bounds__(BEL) void k_a12(Box b, int color, const double *__r
 
 template <int qa, int qb, int qc>
 __device__ __forceinline__ void bf_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
-                                      const double *__restrict__ fe, double F[3][27]) {
+                                      const double *__restrict__ fe, double scale, double F[3][27]) {
   double J[3][3], A[3][3];
   qp_grad<qa>(xr0, xr1, xr2, J);
   const double det = adjugate(J, A);
   constexpr int q = qa + 3 * qb + 9 * qc;
   constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
-  const double s = w * det;
+  const double s = scale * (w * det);
 #pragma unroll
   for (int d = 0; d < 3; d++) F[d][q] = s * fe[3 * q + d];
 }
 
 template <int qb, int qc>
-__device__ __forceinline__ void bf_row(const double *Xs, int base, const double *__restrict__ fe, double F[3][27]) {
+__device__ __forceinline__ void bf_row(const double *Xs, int base, const double *__restrict__ fe, double sc,
+                                       double F[3][27]) {
   double xr0[3][3], xr1[3][3], xr2[3][3];
   row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
-  bf_qp<0, qb, qc>(xr0, xr1, xr2, fe, F);
-  bf_qp<1, qb, qc>(xr0, xr1, xr2, fe, F);
-  bf_qp<2, qb, qc>(xr0, xr1, xr2, fe, F);
+  bf_qp<0, qb, qc>(xr0, xr1, xr2, fe, sc, F);
+  bf_qp<1, qb, qc>(xr0, xr1, xr2, fe, sc, F);
+  bf_qp<2, qb, qc>(xr0, xr1, xr2, fe, sc, F);
 }
 
 __global__ __launch_bounds__(BEL) void k_body_force(Box b, int color, const double *__restrict__ coords,
                                                     const double *__restrict__ fu, double *__restrict__ y,
-                                                    int add_mode) {
+                                                    int add_mode, double sc) {
   __shared__ double Xs[3 * TCS];
   const int lane = threadIdx.x;
   const BrickId br = brick_of(b, color, blockIdx.x);
@@ -856,9 +948,9 @@ __global__ __launch_bounds__(BEL) void k_body_force(Box b, int color, const doub
   if (active) {
     const double *fe = fu + 81 * lane_element(b, t, lane);
     double F[3][27];
-    bf_row<0, 0>(Xs, base, fe, F); bf_row<1, 0>(Xs, base, fe, F); bf_row<2, 0>(Xs, base, fe, F);
-    bf_row<0, 1>(Xs, base, fe, F); bf_row<1, 1>(Xs, base, fe, F); bf_row<2, 1>(Xs, base, fe, F);
-    bf_row<0, 2>(Xs, base, fe, F); bf_row<1, 2>(Xs, base, fe, F); bf_row<2, 2>(Xs, base, fe, F);
+    bf_row<0, 0>(Xs, base, fe, sc, F); bf_row<1, 0>(Xs, base, fe, sc, F); bf_row<2, 0>(Xs, base, fe, sc, F);
+    bf_row<0, 1>(Xs, base, fe, sc, F); bf_row<1, 1>(Xs, base, fe, sc, F); bf_row<2, 1>(Xs, base, fe, sc, F);
+    bf_row<0, 2>(Xs, base, fe, sc, F); bf_row<1, 2>(Xs, base, fe, sc, F); bf_row<2, 2>(Xs, base, fe, sc, F);
     // B^T along x, then y, then z: Y[d][k][j][i] = sum B(qa,i) B(qb,j) B(qc,k) F[d][qa+3qb+9qc]
 #pragma unroll
     for (int d = 0; d < 3; d++) {
@@ -980,14 +1072,15 @@ __global__ __launch_bounds__(BEL) void k_diag(Box b, int color, const double *__
 
 // ------------------------------------------------------------- BC helpers
 // Dirichlet rows: y[idx] = x[idx] (mode 0, BCListInsertDirichlet_MatMult), 0 (mode 1,
-// BCListInsertZero) or 1 (mode 2, BCListInsertValueIntoDirichletSlot).
+// BCListInsertZero), 1 (mode 2, BCListInsertValueIntoDirichletSlot) or x[idx] - v[idx] with v
+// null meaning 0 (mode 3, BCListResidualDirichlet).
 template <typename T>
 __global__ void k_bc_rows(int64_t n, const int64_t *__restrict__ idx, const T *__restrict__ x, T *__restrict__ y,
-                          int mode) {
+                          int mode, const T *__restrict__ v) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int64_t k = idx[i];
-  y[k] = mode == 0 ? x[k] : (mode == 1 ? T(0) : T(1));
+  y[k] = mode == 0 ? x[k] : (mode == 1 ? T(0) : (mode == 2 ? T(1) : x[k] - (v ? v[k] : T(0))));
 }
 
 // Non-finite scan of an input vector (sets ERRBIT_NONFINITE).
@@ -1100,31 +1193,62 @@ hipError_t launch_gather(const Box &b, const T *face, T *y, hipStream_t s) {
 }
 
 hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, const uint8_t *mask,
-                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf, bool gather) {
+                           const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf, bool gather,
+                           const ChebOut<double> *co) {
   if (facebuf) {
-    hipLaunchKernelGGL(k_a11_fp64, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm, y, add ? 1 : 0,
-                       facebuf);
+    if (co) {
+      hipLaunchKernelGGL((k_a11_fp64_t<false, true>), dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm,
+                         y, 0, facebuf, (const double *)nullptr, (double *)nullptr, (const double *)nullptr, *co);
+      return hipGetLastError();
+    }
+    hipLaunchKernelGGL((k_a11_fp64_t<false, false>), dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm, y,
+                       add ? 1 : 0, facebuf, (const double *)nullptr, (double *)nullptr, (const double *)nullptr,
+                       ChebOut<double>{});
     return gather ? launch_gather(b, (const double *)facebuf, y, s) : hipGetLastError();
   }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
-    hipLaunchKernelGGL(k_a11_fp64, dim3(nb), dim3(BEL), 0, s, b, c, coords, u, mask, eta_bm, y, add ? 1 : 0,
-                       (double *)nullptr);
+    hipLaunchKernelGGL((k_a11_fp64_t<false, false>), dim3(nb), dim3(BEL), 0, s, b, c, coords, u, mask, eta_bm, y, add ? 1 : 0,
+                       (double *)nullptr, (const double *)nullptr, (double *)nullptr, (const double *)nullptr,
+                       ChebOut<double>{});
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_stokes_fp64(const Box &b, const double *coords, const double *u, const double *p,
+                              const uint8_t *mask, const double *eta_bm, double *yu, double *yp, const double *fp,
+                              bool add, hipStream_t s, double *facebuf) {
+  if (facebuf) {
+    hipLaunchKernelGGL((k_a11_fp64_t<true, false>), dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, coords, u, mask, eta_bm, yu,
+                       add ? 1 : 0, facebuf, p, yp, fp, ChebOut<double>{});
+    return launch_gather(b, (const double *)facebuf, yu, s);
+  }
+  for (int c = 0; c < 8; c++) {
+    int nb = nbricks_color(b, c);
+    if (nb == 0) continue;
+    hipLaunchKernelGGL((k_a11_fp64_t<true, false>), dim3(nb), dim3(BEL), 0, s, b, c, coords, u, mask, eta_bm, yu,
+                       add ? 1 : 0, (double *)nullptr, p, yp, fp, ChebOut<double>{});
   }
   return hipGetLastError();
 }
 
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
-                           hipStream_t s, float *facebuf, bool gather) {
+                           hipStream_t s, float *facebuf, bool gather, const ChebOut<float> *co) {
   if (facebuf) {
-    hipLaunchKernelGGL(k_a11_fp32, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, fac, y, add ? 1 : 0, facebuf);
+    if (co) {
+      hipLaunchKernelGGL(k_a11_fp32<true>, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, fac, y, 0, facebuf, *co);
+      return hipGetLastError();
+    }
+    hipLaunchKernelGGL(k_a11_fp32<false>, dim3(nbricks(b)), dim3(BEL), 0, s, b, -1, u, mask, fac, y, add ? 1 : 0,
+                       facebuf, ChebOut<float>{});
     return gather ? launch_gather(b, (const float *)facebuf, y, s) : hipGetLastError();
   }
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
-    hipLaunchKernelGGL(k_a11_fp32, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, fac, y, add ? 1 : 0, (float *)nullptr);
+    hipLaunchKernelGGL(k_a11_fp32<false>, dim3(nb), dim3(BEL), 0, s, b, c, u, mask, fac, y, add ? 1 : 0,
+                       (float *)nullptr, ChebOut<float>{});
   }
   return hipGetLastError();
 }
@@ -1200,11 +1324,11 @@ hipError_t launch_a12(const Box &b, const double *coords, const double *p, doubl
 }
 
 hipError_t launch_body_force(const Box &b, const double *coords, const double *fu, double *y, bool add,
-                             hipStream_t s) {
+                             hipStream_t s, double scale) {
   for (int c = 0; c < 8; c++) {
     int nb = nbricks_color(b, c);
     if (nb == 0) continue;
-    hipLaunchKernelGGL(k_body_force, dim3(nb), dim3(BEL), 0, s, b, c, coords, fu, y, add ? 1 : 0);
+    hipLaunchKernelGGL(k_body_force, dim3(nb), dim3(BEL), 0, s, b, c, coords, fu, y, add ? 1 : 0, scale);
   }
   return hipGetLastError();
 }
@@ -1219,13 +1343,15 @@ hipError_t launch_diag(const Box &b, const double *coords, const double *eta_bm,
 }
 
 template <typename T>
-hipError_t launch_bc_rows(int64_t n, const int64_t *idx, const T *x, T *y, int mode, hipStream_t s) {
+hipError_t launch_bc_rows(int64_t n, const int64_t *idx, const T *x, T *y, int mode, hipStream_t s, const T *v) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bc_rows<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, idx, x, y, mode);
+  hipLaunchKernelGGL(k_bc_rows<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, idx, x, y, mode, v);
   return hipGetLastError();
 }
-template hipError_t launch_bc_rows<double>(int64_t, const int64_t *, const double *, double *, int, hipStream_t);
-template hipError_t launch_bc_rows<float>(int64_t, const int64_t *, const float *, float *, int, hipStream_t);
+template hipError_t launch_bc_rows<double>(int64_t, const int64_t *, const double *, double *, int, hipStream_t,
+                                           const double *);
+template hipError_t launch_bc_rows<float>(int64_t, const int64_t *, const float *, float *, int, hipStream_t,
+                                          const float *);
 
 template <typename T>
 hipError_t launch_scan_finite(int64_t n, const T *x, int *err, hipStream_t s) {

@@ -91,11 +91,14 @@ __global__ void k_cheb_first(int64_t n, const T *__restrict__ b, const T *__rest
 // apply + update.  Dirichlet dofs take y = x (the apply input; MatMult semantics).  Then
 //   out = cr * dinv * (b - y) + ck * pkm1 + cx * x       (pkm1 may be null: first step).
 // The strided read-modify-write of the x faces that dominates the separate pass goes away.
+// face_only: the apply's store already updated the Dirichlet dofs and every dof that no face
+// slot adds to (store_tile_own<true>); only the lower-face dofs are left.
 template <typename T>
 __global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T *__restrict__ y,
                              const T *__restrict__ dinv, const T *__restrict__ x, const T *__restrict__ pkm1,
                              T *__restrict__ out, const T *__restrict__ face, const uint8_t *__restrict__ mask, T cr,
-                             T ck, T cx) {
+                             T ck, T cx, int face_only) {
+  const ptq2::ChebOut<T> co{b, dinv, x, pkm1, out, mask, cr, ck, cx};
   constexpr int TN = ptq2::TN, F2 = TN * TN, UP = 2 * ptq2::BE, NB = 2 * ptq2::BE;
   const unsigned NX = (unsigned)bx.NX, NXY = (unsigned)bx.NX * (unsigned)bx.NY;
   const int64_t sx = ptq2::FACEVALS, sy = (int64_t)bx.nbx * ptq2::FACEVALS,
@@ -103,6 +106,7 @@ __global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     T yv;
     if (mask && mask[i]) {
+      if (face_only) continue;
       yv = x[i];
     } else {
       yv = y[i];
@@ -113,6 +117,7 @@ __global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T
                   bzi = min((int)(Z / NB), bx.nbz - 1);
         const int lx = (int)X - NB * bxi, ly = (int)Yn - NB * byi, lz = (int)Z - NB * bzi;
         const bool nx = lx == 0 && bxi > 0, ny = ly == 0 && byi > 0, nz = lz == 0 && bzi > 0;
+        if (face_only && !(nx || ny || nz)) continue;
         if (nx || ny || nz) {
           const T *fb = face + ((int64_t)bxi + (int64_t)bx.nbx * (byi + (int64_t)bx.nby * bzi)) * ptq2::FACEVALS;
           T acc = yv;
@@ -127,10 +132,7 @@ __global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T
         }
       }
     }
-    if (pkm1)
-      out[i] = cr * dinv[i] * (b[i] - yv) + ck * pkm1[i] + cx * x[i];
-    else
-      out[i] = cr * dinv[i] * (b[i] - yv) + x[i];
+    out[i] = ptq2::cheb_update(co, i, yv);
   }
 }
 
@@ -1061,6 +1063,24 @@ int level_apply_owned<float>(ptq2_mg *mg, int l, const float *x, float *y) {
   return PTQ2_OK;
 }
 
+// the apply with the Chebyshev update of co fused into its store (owned nodes without face
+// contributions); the rest is left in y for k_cheb_faces(face_only)
+template <typename T>
+int level_apply_cheb(ptq2_mg *mg, int l, const T *x, T *y, const ptq2::ChebOut<T> &co);
+template <>
+int level_apply_cheb<double>(ptq2_mg *mg, int l, const double *x, double *y, const ptq2::ChebOut<double> &co) {
+  ptq2_ctx *c = mg->ctx[l];
+  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream, (double *)c->facebuf,
+                              false, &co));
+  return PTQ2_OK;
+}
+template <>
+int level_apply_cheb<float>(ptq2_mg *mg, int l, const float *x, float *y, const ptq2::ChebOut<float> &co) {
+  ptq2_ctx *c = mg->ctx[l];
+  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, false, &co));
+  return PTQ2_OK;
+}
+
 // ChebychevRN on level l: x <- smoothed solution of A x = b; zero_guess skips the first residual
 template <typename T>
 int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
@@ -1077,24 +1097,33 @@ int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
   c[k] = mu;
   // apply (owner values + face slots) and update in one pass over the vectors
   const ptq2_ctx *lc = mg->ctx[l];
-  auto fused = [&](const T *xin, const T *pkm1, T *out, double cr, double ck, double cx) {
+  // one Chebyshev step from xin: the apply, then the update.  Fused: the apply's store
+  // updates the final owned dofs and the face pass the lower-face ones; otherwise the apply
+  // stores A xin and the face pass updates every dof.
+  const bool fz = mg->opt.fused_smoother != 0 && lc->facebuf;
+  auto step = [&](const T *xin, const T *pkm1, T *out, double cr, double ck, double cx) -> int {
+    const ptq2::ChebOut<T> co{(const T *)b, (const T *)dinv, xin, pkm1, out, (const uint8_t *)lc->mask, (T)cr, (T)ck,
+                              (T)cx};
+    if (fz)
+      MGRC(level_apply_cheb<T>(mg, l, xin, y, co));
+    else
+      MGRC(level_apply_owned<T>(mg, l, xin, y));
     hipLaunchKernelGGL(k_cheb_faces<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, lc->box, n, (const T *)b,
                        (const T *)y, (const T *)dinv, xin, pkm1, out, (const T *)lc->facebuf,
-                       (const uint8_t *)lc->mask, (T)cr, (T)ck, (T)cx);
+                       (const uint8_t *)lc->mask, (T)cr, (T)ck, (T)cx, fz ? 1 : 0);
+    return PTQ2_OK;
   };
   if (zero_guess) {
     hipLaunchKernelGGL(k_cheb_first<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const T *)b, (const T *)nullptr,
                        (const T *)dinv, (const T *)nullptr, buf[k], (T)scale);
     hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, x, T(0));
   } else {
-    MGRC(level_apply_owned<T>(mg, l, x, y));
-    fused(x, nullptr, buf[k], scale, 0.0, 1.0);
+    MGRC(step(x, nullptr, buf[k], scale, 0.0, 1.0));
   }
   for (int i = 0; i < mg->opt.smooth_its; i++) {
     c[kp1] = 2.0 * mu * c[k] - c[km1];
     const double omega = omegaprod * c[k] / c[kp1];
-    MGRC(level_apply_owned<T>(mg, l, buf[k], y));
-    fused(buf[k], buf[km1], buf[kp1], omega * scale, 1.0 - omega, omega);
+    MGRC(step(buf[k], buf[km1], buf[kp1], omega * scale, 1.0 - omega, omega));
     int t = km1;
     km1 = k;
     k = kp1;
@@ -1709,6 +1738,7 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->rigid_deflation = 0;
   o->rigid_min_contrast = 100.0;
   o->rigid_max_inclusions = RG_MAXINC;
+  o->fused_smoother = 0;
 }
 
 int ptq2_mg_destroy(ptq2_mg **pmg) {

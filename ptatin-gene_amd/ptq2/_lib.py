@@ -48,6 +48,8 @@ SIGNATURES = {
     "ptq2_a21_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_stokes_apply": (_c_int, [_vp, _vp, _vp, _vp, _vp, _c_uint]),
     "ptq2_sizes": (_c_int, [_vp, ctypes.POINTER(_i64), ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "ptq2_stokes_residual": (_c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _c_uint]),
+    "ptq2_project_points_q1": (_c_int, [_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _c_uint]),
 }
 
 _lib = None

@@ -191,6 +191,44 @@ def body_force(mx, my, mz, coords, fu):
     return out.reshape(-1)
 
 
+def point_lattice(mx, my, mz, coords, n=(2, 2, 2)):
+    """Material points on the zero-perturbation lattice layout: n[0]*n[1]*n[2] per element,
+    element-major and x fastest, local coordinates xi_d = -1 + (2/n_d)(p_d + 1/2), physical
+    coordinates by Q2 interpolation of the element's nodes
+    (SwarmMPntStd_CoordAssignment_LatticeLayout3d, src/material_point_std_utils.c:100-210, then
+    MaterialPointCoordinateSetUp, src/ptatin3d.c:520-562).  Returns xi (P, 3), wil (P,), x (P, 3)."""
+    E = mx * my * mz
+    g = [-1.0 + (2.0 / n[d]) * (np.arange(n[d]) + 0.5) for d in range(3)]
+    loc = np.stack(np.meshgrid(g[0], g[1], g[2], indexing="ij"), -1).transpose(2, 1, 0, 3).reshape(-1, 3)
+    npc = loc.shape[0]
+
+    def l3(x):
+        return np.stack([0.5 * x * (x - 1.0), (1.0 + x) * (1.0 - x), 0.5 * (1.0 + x) * x], -1)
+
+    N = np.einsum("pk,pj,pi->pkji", l3(loc[:, 2]), l3(loc[:, 1]), l3(loc[:, 0])).reshape(npc, 27)
+    nx, ny = 2 * mx + 1, 2 * my + 1
+    e = np.arange(E)
+    ex, ey, ez = e % mx, (e // mx) % my, e // (mx * my)
+    off = np.array([i + nx * (j + ny * k) for k in range(3) for j in range(3) for i in range(3)])
+    nodes = (2 * ex + nx * (2 * ey + ny * 2 * ez))[:, None] + off[None, :]
+    c = np.asarray(coords, dtype=np.float64).reshape(-1, 3)
+    x = np.einsum("pn,end->epd", N, c[nodes]).reshape(-1, 3)
+    xi = np.tile(loc, (E, 1))
+    wil = np.repeat(e, npc).astype(np.int32)
+    return xi, wil, x
+
+
+def sinker_single_inclusion(x, eta0=1.0, eta1=10.0, rho0=0.0, rho1=1.0, origin=(0.5, 0.5, 0.5),
+                            length=(0.5, 0.5, 0.5)):
+    """Point properties of the viscous sinker's single spherical inclusion
+    (ViscousSinker_ApplyInitialMaterialGeometry_SingleInclusion,
+    src/models/viscous_sinker/model_viscous_sinker_ops.c:760-838): inside when
+    sum_d ((x_d - o_d) / (l_d / 2))^2 < 1.  Returns eta, rho (rho = -rho1 inside, -rho0 outside)."""
+    r = (((np.asarray(x) - np.asarray(origin)) / (0.5 * np.asarray(length))) ** 2).sum(axis=1)
+    inside = r < 1.0
+    return np.where(inside, eta1, eta0).astype(np.float64), np.where(inside, -rho1, -rho0).astype(np.float64)
+
+
 def sphere_inclusions(n, rmin, rmax, seed, lo=0.0, hi=1.0):
     """n spheres, radius U[rmin,rmax], centre U[lo+r, hi-r]^3 (SplitMix64 stream `seed`)."""
     u = uniform(seed, 4 * n).reshape(n, 4)

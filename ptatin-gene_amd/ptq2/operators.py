@@ -90,6 +90,63 @@ class Context:
             check(_lib.lib().ptq2_set_viscosity(self.handle, eta.ctypes.data, state, 0),
                   "ptq2_set_viscosity", self.handle)
 
+    def residual(self, u, p, fu, fp=None, phi=None, Fu=None, Fp=None):
+        """FormFunction_Stokes (Fu, Fp): momentum A11 u_loc + A12 p - Be(fu) with Dirichlet rows
+        u - phi, continuity A21 u_loc - int N^p fp (u_loc = u with the Dirichlet values phi)."""
+        L = _lib.lib()
+        if _is_dev(u):
+            if Fu is None:
+                Fu = torch.empty(self.nvel, dtype=torch.float64, device=u.device)
+            if Fp is None:
+                Fp = torch.empty(self.npres, dtype=torch.float64, device=u.device)
+            self._stream(u)
+            ptr = (lambda t: None if t is None else t.data_ptr())
+            check(L.ptq2_stokes_residual(self.handle, u.data_ptr(), p.data_ptr(), fu.data_ptr(), ptr(fp), ptr(phi),
+                                         Fu.data_ptr(), Fp.data_ptr(), DEVICE), "ptq2_stokes_residual", self.handle)
+            return Fu, Fp
+        arrs = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1) for a in (u, p, fu, fp, phi)]
+        Fu = np.zeros(self.nvel) if Fu is None else Fu
+        Fp = np.zeros(self.npres) if Fp is None else Fp
+        ptrs = [None if a is None else a.ctypes.data for a in arrs]
+        check(L.ptq2_stokes_residual(self.handle, *ptrs, Fu.ctypes.data, Fp.ctypes.data, 0), "ptq2_stokes_residual",
+              self.handle)
+        return Fu, Fp
+
+    def project_points(self, xi, wil, eta_p, rho_p=None, eta_qp=None, rho_qp=None):
+        """Q1 projection of material-point values onto the quadrature points
+        (SwarmUpdateGaussPropertiesLocalL2Projection_Q1_MPntPStokes); returns eta_qp, or
+        (eta_qp, rho_qp) when rho_p is given.  numpy arrays in, numpy out; torch device
+        tensors in, device tensors out."""
+        L = _lib.lib()
+        n27 = 27 * self.nel
+        if _is_dev(eta_p):
+            dev = eta_p.device
+            xi = xi.reshape(-1).contiguous()
+            wil = wil.to(torch.int32).contiguous()
+            if eta_qp is None:
+                eta_qp = torch.empty(n27, dtype=torch.float64, device=dev)
+            if rho_p is not None and rho_qp is None:
+                rho_qp = torch.empty(n27, dtype=torch.float64, device=dev)
+            self._stream(eta_p)
+            ptr = (lambda t: None if t is None else t.data_ptr())
+            check(L.ptq2_project_points_q1(self.handle, wil.numel(), xi.data_ptr(), wil.data_ptr(), eta_p.data_ptr(),
+                                           ptr(rho_p), eta_qp.data_ptr(), ptr(rho_qp), DEVICE),
+                  "ptq2_project_points_q1", self.handle)
+        else:
+            xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1)
+            wil = np.ascontiguousarray(wil, dtype=np.int32)
+            eta_p = np.ascontiguousarray(eta_p, dtype=np.float64)
+            eta_qp = np.zeros(n27) if eta_qp is None else eta_qp
+            rp = None
+            if rho_p is not None:
+                rho_p = np.ascontiguousarray(rho_p, dtype=np.float64)
+                rp = rho_p.ctypes.data
+                rho_qp = np.zeros(n27) if rho_qp is None else rho_qp
+            check(L.ptq2_project_points_q1(self.handle, wil.size, xi.ctypes.data, wil.ctypes.data, eta_p.ctypes.data,
+                                           rp, eta_qp.ctypes.data, None if rho_qp is None else rho_qp.ctypes.data, 0),
+                  "ptq2_project_points_q1", self.handle)
+        return eta_qp if rho_p is None else (eta_qp, rho_qp)
+
     def sync(self):
         check(_lib.lib().ptq2_sync(self.handle), "ptq2_sync", self.handle)
 

@@ -18,7 +18,8 @@ class MGOpts(ctypes.Structure):
                 ("emin_frac", ctypes.c_double), ("emax_frac", ctypes.c_double), ("esteig_its", ctypes.c_int),
                 ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int), ("galerkin_coarse", ctypes.c_int),
                 ("coarse_eta_mean", ctypes.c_int), ("fp32_basis", ctypes.c_int), ("rigid_deflation", ctypes.c_int),
-                ("rigid_min_contrast", ctypes.c_double), ("rigid_max_inclusions", ctypes.c_int)]
+                ("rigid_min_contrast", ctypes.c_double), ("rigid_max_inclusions", ctypes.c_int),
+                ("fused_smoother", ctypes.c_int)]
 
 ETA_MEANS = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "max": 3, "q1proj": 4}
 
@@ -71,7 +72,7 @@ class ViscousSolver:
 
     def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000,
                  galerkin_coarse=True, eta_mean="arithmetic", emin_frac=None, emax_frac=None, rigid_deflation=False,
-                 rigid_min_contrast=None, rigid_max_inclusions=None, fp32_basis=False):
+                 rigid_min_contrast=None, rigid_max_inclusions=None, fp32_basis=False, fused_smoother=None):
         L = _sl()
         o = default_opts()
         o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
@@ -85,6 +86,8 @@ class ViscousSolver:
         if emax_frac is not None:
             o.emax_frac = emax_frac
         o.fp32_basis = 1 if fp32_basis else 0
+        if fused_smoother is not None:
+            o.fused_smoother = 1 if fused_smoother else 0
         o.rigid_deflation = 1 if rigid_deflation else 0
         if rigid_min_contrast is not None:
             o.rigid_min_contrast = rigid_min_contrast

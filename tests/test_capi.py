@@ -14,3 +14,24 @@ def test_c_consumer(tmp_path):
     out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr + out.stdout
     assert "capi ok" in out.stdout
+
+
+def test_fast_tile_addressing(tmp_path):
+    """The full-tile fast path of the kernels' tile loads and stores addresses exactly the rows
+    of the general path (host compile of ptq2_device.h's address helpers)."""
+    exe = str(tmp_path / "test_tile_addressing")
+    subprocess.check_call(["/opt/rocm/bin/hipcc", "--offload-arch=gfx950", "-O1", "-std=c++17", "-I", os.path.join(PKG, "csrc"),
+                           os.path.join(REPO, "tests", "c", "test_tile_addressing.cpp"), "-o", exe])
+    out = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr + out.stdout
+    assert "tile addressing ok" in out.stdout
+
+
+def test_petsc_adapter_compiles_against_stub():
+    """The -a11_op hip adapter (ptatin-gene_amd/adapter/stokes_operators_hip.c) type-checks as C99
+    against a minimal stand-in of the PETSc / pTatin3D declarations it uses
+    (tests/c/petsc_stub/): every libptq2 call matches include/ptq2.h."""
+    subprocess.check_call(["gcc", "-std=c99", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                           "-Wno-unused-parameter", "-I", os.path.join(REPO, "tests", "c", "petsc_stub"),
+                           "-I", os.path.join(REPO, "include"),
+                           os.path.join(PKG, "adapter", "stokes_operators_hip.c")])

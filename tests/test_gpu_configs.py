@@ -165,14 +165,20 @@ def test_c5_batch_concurrent_equals_sequential():
         nb = np.linalg.norm(b)
         ax_oracle = oracle_matmult(P, xh)
         r = np.linalg.norm(b - ax_oracle) / nb
-        # The residual of one x is only defined to the rounding of the operator's element sums,
-        # which grows with the contrast (stiff elements contribute ~eta_in times the load).  The
-        # floor is the library's and the oracle's FP64 A x apart, measured on this x (host path:
-        # synchronous on the context's own stream).
-        floor = np.linalg.norm(ctx.a11(xh, flags=ptq2.MATMULT) - ax_oracle) / nb
-        say("  %d^3 oracle rel residual %.3e, FP64 residual floor %.2e" % (P.mx, r, floor))
-        assert floor <= 2e-9
-        assert r <= 1e-8 + floor + 1e-12
+        # The residual of one x is only defined to the rounding of the operator's element sums:
+        # inside a stiff inclusion x is close to a rigid motion, whose element stresses cancel
+        # only to ~eps * eta_in, so any two FP64 evaluations of A x disagree there.  The oracle's
+        # own two restatements (sum-factorized and per-point, different formulas) measure that
+        # floor on this x; the library (host path: synchronous on the context's own stream) may
+        # sit at most 10x further from the sum-factorized oracle than that.
+        gap = np.linalg.norm(ctx.a11(xh, flags=ptq2.MATMULT) - ax_oracle) / nb
+        ax_ref = O.a11_ref(P.mx, P.my, P.mz, P.coords, P.eta, np.where(P.mask > 0, 0.0, xh))
+        ax_ref[P.mask > 0] = xh[P.mask > 0]
+        pair = np.linalg.norm(ax_ref - ax_oracle) / nb
+        say("  %d^3 oracle rel residual %.3e; |A_lib x - A_oracle x|/|b| %.2e, oracle tensor vs per-point "
+            "forms %.2e" % (P.mx, r, gap, pair))
+        assert gap <= 10.0 * pair + 1e-12
+        assert r <= 1e-8 + gap + 1e-12
         S.destroy()
         ctx.destroy()
 

@@ -176,6 +176,39 @@ def test_full_block(sinker):
 
 
 @pytest.mark.parametrize("dims", [(12, 12, 12), (9, 6, 5)])
+def test_full_block_fp64_one_pass(dims):
+    """FP64 full block in one element pass against the oracle (MFStokesWrapper_A_AVX form) on a
+    jittered contrast-1e6 sinker with partial bricks: MatMult 1e-12; raw ADD mode adds onto the
+    output; the resident-geometry variant (three passes) agrees; repeated calls are bitwise
+    identical."""
+    mx, my, mz = dims
+    say("case: FP64 full block %s contrast 1e6" % (dims,))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 2)
+    cen, r = M.sphere_inclusions(8, 0.1, 0.2, 6)
+    eta = M.viscosity_inclusions(M.qp_coords(mx, my, mz, c), cen, r, 1e6, 1.0, None)
+    mask = M.bc_mask(mx, my, mz, "freeslip")
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    ctx.set_viscosity(eta)
+    nvel, npres = 3 * M.nnodes(mx, my, mz), 4 * mx * my * mz
+    u = M.uniform(11, nvel, -1.0, 1.0)
+    p = M.uniform(12, npres, -1.0, 1.0)
+    ru, rp = O.matmult_a(mx, my, mz, c, eta, mask, u, p)
+    yu, yp = ctx.stokes(u, p, flags=_lib.MATMULT)
+    assert rel(yu, ru) <= 1e-12 and rel(yp, rp) <= 1e-12
+    yu2, yp2 = ctx.stokes(u, p, flags=_lib.MATMULT)
+    assert np.array_equal(yu, yu2) and np.array_equal(yp, yp2)
+    gu, gp = ctx.stokes(u, p, flags=_lib.MATMULT | _lib.STOREDGEO)
+    assert rel(gu, ru) <= 1e-12 and rel(gp, rp) <= 1e-12
+    zu, zp = O.stokes_a(mx, my, mz, c, eta, u, p)
+    au = M.uniform(13, nvel, -1.0, 1.0)
+    ap = M.uniform(14, npres, -1.0, 1.0)
+    wu, wp = ctx.stokes(u, p, au.copy(), ap.copy(), flags=_lib.ADD)
+    assert rel(wu, zu + au) <= 1e-12 and rel(wp, zp + ap) <= 1e-12
+    ctx.destroy()
+
+
+@pytest.mark.parametrize("dims", [(12, 12, 12), (9, 6, 5)])
 def test_full_block_fp32(dims):
     """FP32 full block (one fused pass, FP32-stored factors) against the FP64 oracle on a
     sinker with viscosity contrast 1e6, MatMult semantics with free-slip walls; raw ADD mode
@@ -340,17 +373,17 @@ def test_full_size_properties_128():
 
 
 @pytest.mark.parametrize("dims", [(8, 9, 11), (16, 16, 16), (5, 7, 14)])
-def test_body_force_matches_host_restatement(dims):
-    """ptq2_body_force against the host restatement ptq2.mesh.body_force (pinned by pointwise
-    assembly and by the integral identity in tests/test_host.py) on jittered meshes with partial
-    bricks: FP64 1e-12; host and device pointers bitwise equal; ADD accumulates; MATMULT zeroes
-    the Dirichlet rows; a non-finite input is reported."""
+def test_body_force_matches_oracle(dims):
+    """ptq2_body_force against the oracle's Be loop (ptq2o_body_force, per-point form of
+    FormFunctionLocal_U; its pins are in tests/test_oracle.py and tests/test_host.py) on jittered
+    meshes with partial bricks: FP64 1e-12; host and device pointers bitwise equal; ADD
+    accumulates; MATMULT zeroes the Dirichlet rows; a non-finite input is reported."""
     mx, my, mz = dims
     say("case: body force %dx%dx%d" % dims)
     c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 3)
     mask = M.bc_mask(mx, my, mz, "freeslip")
     fu = M.uniform(31, 81 * mx * my * mz, -1, 1).reshape(-1, 27, 3)
-    ref = M.body_force(mx, my, mz, c, fu)
+    ref = O.body_force(mx, my, mz, c, fu)
     ctx = ptq2.Context(0)
     ctx.setup_mesh(mx, my, mz, c, mask)
     y = ctx.body_force(fu)
@@ -367,4 +400,44 @@ def test_body_force_matches_host_restatement(dims):
     bad[3, 4, 1] = np.nan
     with pytest.raises(ptq2.PTQ2Error):
         ctx.body_force(bad)
+    ctx.destroy()
+
+
+@pytest.mark.parametrize("dims,with_fp", [((8, 9, 11), True), ((16, 16, 16), False), ((5, 7, 14), True)])
+def test_stokes_residual_matches_oracle(dims, with_fp):
+    """ptq2_stokes_residual (FormFunctionLocal_U / _P: A11 u_loc + A12 p - Be, A21 u_loc - int N^p
+    fp, Dirichlet rows u - phi) against the oracle's residual on jittered meshes with partial
+    bricks, nonzero Dirichlet values and a variable viscosity: FP64 1e-12 relative in each part;
+    host and device paths bitwise equal."""
+    mx, my, mz = dims
+    say("case: residual %dx%dx%d fp=%s" % (mx, my, mz, with_fp))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 7)
+    mask = M.bc_mask(mx, my, mz, "freeslip_freesurface")
+    ne = mx * my * mz
+    eta = 10.0 ** M.uniform(60, 27 * ne, -1, 2)
+    nv = 3 * M.nnodes(mx, my, mz)
+    u = M.uniform(61, nv, -1, 1)
+    p = M.uniform(62, 4 * ne, -1, 1)
+    fu = M.uniform(63, 81 * ne, -1, 1)
+    fp = M.uniform(64, 27 * ne, -1, 1) if with_fp else None
+    phi = M.uniform(65, nv, -0.5, 0.5)
+    ref_u, ref_p = O.residual(mx, my, mz, c, eta, mask, u, p, fu, fp, phi)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    ctx.set_viscosity(eta)
+    Fu, Fp = ctx.residual(u, p, fu, fp, phi)
+    eu, ep = rel(Fu, ref_u), rel(Fp, ref_p)
+    say("  Fu rel err %.3e, Fp rel err %.3e" % (eu, ep))
+    assert eu <= 1e-12 and ep <= 1e-12
+    d = mask > 0
+    assert np.array_equal(Fu[d], u[d] - phi[d])
+    t = [None if a is None else torch.from_numpy(a).to(DEV) for a in (u, p, fu, fp, phi)]
+    Fud, Fpd = ctx.residual(*t)
+    torch.cuda.synchronize()
+    assert np.array_equal(Fud.cpu().numpy(), Fu) and np.array_equal(Fpd.cpu().numpy(), Fp)
+    # zero Dirichlet values (phi = None) and no sources: the block MatMult on u with zeroed
+    # Dirichlet columns, except the Dirichlet rows (u_D here)
+    Fu0, Fp0 = ctx.residual(u, p, np.zeros(81 * ne))
+    yu, yp = O.matmult_a(mx, my, mz, c, eta, mask, u, p)
+    assert rel(Fu0[~d], yu[~d]) <= 1e-12 and rel(Fp0, yp) <= 1e-12
     ctx.destroy()
