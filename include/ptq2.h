@@ -92,6 +92,15 @@ int ptq2_set_viscosity(ptq2_ctx *ctx, const double *eta_qp, uint64_t state, unsi
    ufield[] local velocity in, element contributions summed into Yu[].
    PTQ2_MATMULT is MatMult_MFStokes_A11 (src/stokes_operators.c:1156-1222). */
 int ptq2_a11_apply(ptq2_ctx *ctx, const void *u, void *y, unsigned flags);
+/* No counterpart in the reference.  ptq2_a11_apply for callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+   the same operator and results, FP64 with the geometry recomputed from the coordinates; flags
+   must hold PTQ2_DEVICE and may add PTQ2_MATMULT or PTQ2_ADD.  The face slots of the one-launch
+   output path live in a library-owned static device array of 32 MiB per GPU instead of a
+   per-context buffer, so a mesh above 5753 bricks of 4x4x4 elements (e.g. beyond 68^3 elements)
+   is PTQ2_ERR_SIZE.  Calls that overlap on one GPU (any stream, any context) race on that array;
+   the library never allocates in this call.  Stream ordered, like the other PTQ2_DEVICE calls. */
+int ptq2_a11_apply_noalloc(ptq2_ctx *ctx, const double *u, double *y, unsigned flags);
 /* diagonal of A11: MFStokesWrapper_diagA11 (src/stokes_operators_mf.c:71-135);
    with PTQ2_MATMULT, MatGetDiagonal_MFStokes_A11 (src/stokes_operators.c:1072-1113)
    which puts 1 in Dirichlet slots.  FP64 only. */

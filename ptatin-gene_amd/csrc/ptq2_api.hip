@@ -13,6 +13,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -24,6 +25,32 @@
 using ptq2::Box;
 
 namespace {
+
+// Face slots of ptq2_a11_apply_noalloc: a library-owned device array instead of a per-context
+// buffer, so that the call allocates nothing.  Each GPU holds its own copy from the first use of
+// the library on it until the process ends.  Host code never names the array in a launch or copy:
+// its host-side symbol is not a device address.  ptq2_create resolves the device address once
+// per device (hipGetSymbolAddress with that device current) and the entry point uses that.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(32) << 20;  // 32 MiB per GPU
+__device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
+constexpr int NOALLOC_MAXDEV = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[NOALLOC_MAXDEV] = {};  // device address of `scratch` per device, or null
+
+// the device address of `scratch` on the current device `dev` (resolved on first call)
+hipError_t scratch_resolve(int dev, double **out) {
+  *out = nullptr;
+  if (dev < 0 || dev >= NOALLOC_MAXDEV) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[dev] = (double *)ptr;
+  }
+  *out = g_scratch[dev];
+  return hipSuccess;
+}
 
 int fail(ptq2_ctx *c, int code, const char *fmt, ...) {
   if (c) {
@@ -192,6 +219,13 @@ int ptq2_create(ptq2_ctx **out, int device) {
     return PTQ2_ERR_DEVICE;
   }
   c->stream = c->own_stream;
+  double *scr = nullptr;
+  if (scratch_resolve(device, &scr) != hipSuccess) {  // the device is current (hipSetDevice above)
+    (void)hipGetLastError();
+    (void)hipStreamDestroy(c->own_stream);
+    delete c;
+    return PTQ2_ERR_DEVICE;
+  }
   if (hipMalloc((void **)&c->err, sizeof(int)) != hipSuccess) {
     (void)hipGetLastError();
     (void)hipStreamDestroy(c->own_stream);
@@ -419,6 +453,29 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
     if ((rc = read_err(c, &bits))) return rc;
     if (bits & ptq2::ERRBIT_NONFINITE) return fail(c, PTQ2_ERR_NONFINITE, "non-finite value in the input vector");
   }
+  return PTQ2_OK;
+}
+
+int ptq2_a11_apply_noalloc(ptq2_ctx *c, const double *u, double *y, unsigned flags) {
+  int rc = check_ready(c, true);
+  if (rc) return rc;
+  if (!u || !y) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
+  if (!(flags & PTQ2_DEVICE)) return fail(c, PTQ2_ERR_INVALID_ARG, "PTQ2_DEVICE is required (host vectors need staging)");
+  if (flags & ~(PTQ2_DEVICE | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY))
+    return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags (FP64, geometry from the coordinates only)");
+  if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
+  const size_t need = sizeof(double) * (size_t)c->nbricks * ptq2::FACEVALS;
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(c, PTQ2_ERR_SIZE, "mesh of %lld bricks needs %zu B of face slots, above the %zu B scratch",
+                (long long)c->nbricks, need, NOALLOC_SCRATCH_BYTES);
+  if ((rc = set_device(c))) return rc;
+  double *scr = nullptr;
+  HIPCHK(c, scratch_resolve(c->device, &scr));  // resolved by ptq2_create: a table lookup
+  const bool mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD;
+  // every face slot the gather pass reads is written by the apply launch before it in stream
+  // order, so the array needs no clearing between calls
+  HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, u, mm ? c->mask : nullptr, c->eta_bm, y, add, c->stream, scr));
+  if (mm) HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, u, y, 0, c->stream));
   return PTQ2_OK;
 }
 

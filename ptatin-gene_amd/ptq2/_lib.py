@@ -42,6 +42,7 @@ SIGNATURES = {
                                      _c_int, _c_int, _c_int, _vp, _vp]),
     "ptq2_set_viscosity": (_c_int, [_vp, _vp, ctypes.c_uint64, _c_uint]),
     "ptq2_a11_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
+    "ptq2_a11_apply_noalloc": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_a11_diag": (_c_int, [_vp, _vp, _c_uint]),
     "ptq2_a12_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_body_force": (_c_int, [_vp, _vp, _vp, _c_uint]),
