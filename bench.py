@@ -201,7 +201,7 @@ def cpu_baseline(seconds=8.0):
 
 
 def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=0.05, rigid=False,
-                     variants=("fp64", "fp32", "mixed")):
+                     variants=("fp64", "fp32", "mixed"), fused_smoother=None, autotune=None):
     """The viscous solve of the buoyancy problem (free-slip walls, 20 dense spheres with
     viscosity contrast 1e4) with GMG-preconditioned FGMRES: FP64 throughout ("fp64"), FP32
     Chebyshev smoothers ("fp32"), and FP32 smoothers with the finished Arnoldi vectors stored in
@@ -233,11 +233,20 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
             continue
         t = time.perf_counter()
         S = ViscousSolver(ctx, nlevels=nlevels, smooth_its=4, fp32=fp32, emin_frac=emin_frac, rigid_deflation=rigid,
-                          fp32_basis=b32)
+                          fp32_basis=b32, fused_smoother=fused_smoother, autotune=autotune)
         torch.cuda.synchronize()
         setup = time.perf_counter() - t
         S.vcycle(bd)  # warm the graph
         torch.cuda.synchronize()
+        if key != "mixed":  # (the mixed solve has fp32's hierarchy)
+            out[key + "_tuning"] = [{"kernel": ViscousSolver.KERNELS[k], "fast_tiles": f, "ms": round(a, 4),
+                                     "ms_default": round(d, 4)} for k, f, a, d in S.tuning()]
+            out[key + "_level_ms"] = [round(v, 4) for v in S.level_times(bd, reps=5)]
+            t0 = time.perf_counter()
+            for _ in range(5):
+                S.vcycle(bd)
+            torch.cuda.synchronize()
+            out[key + "_vcycle_ms"] = round((time.perf_counter() - t0) / 5 * 1e3, 4)
         t = time.perf_counter()
         x, info = S.solve(bd, rtol=rtol, max_it=max_it, restart=restart)
         torch.cuda.synchronize()
@@ -320,6 +329,48 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
     return out
 
 
+def batch_apply_comparison(steps, warmup, stream, sizes=(24, 32, 40, 48, 56, 64, 64, 80)):
+    """A11 of the 8 uneven sinker problems of the batched solves (viscosity contrasts 1e2..1e6):
+    the 8 single applies one after another on one stream against one batched launch
+    (ptq2_a11_apply_batch), FP64 and FP32.  Time per round of 8 applies."""
+    import torch
+
+    import ptq2
+    from ptq2 import mesh as M
+
+    contrasts = [10.0 ** (2 + 4 * k / max(1, len(sizes) - 1)) for k in range(len(sizes))]
+    ctxs, us = [], []
+    for k, (m, c) in enumerate(zip(sizes, contrasts)):
+        P, _ = M.config_buoyancy(m, contrast=c, seed_spheres=100 + k)
+        ctx = ptq2.Context(0)
+        ctx.setup_mesh(m, m, m, P.coords, P.mask)
+        ctx.set_viscosity(P.eta)
+        ctxs.append(ctx)
+        us.append(torch.from_numpy(M.uniform(5 + k, ctx.nvel, -1.0, 1.0)).to("cuda:0"))
+    ndofs = sum(c.nvel for c in ctxs)
+    out = {"workload": "A11 of 8 problems %s^3 (buoyancy sinkers, contrasts 1e2..1e6), velocity dofs %d"
+                       % ("/".join(str(m) for m in sizes), ndofs)}
+    for name, flags, conv in (("fp64", 0, lambda t: t), ("fp32", ptq2.FP32, lambda t: t.float())):
+        uu = [conv(u) for u in us]
+        ys = [torch.empty_like(u) for u in uu]
+        for c, u, y in zip(ctxs, uu, ys):
+            c.a11(u, y, flags | ptq2.DEVICE)  # FP32 factors and face buffers outside the timing
+        torch.cuda.synchronize()
+
+        def single():
+            for c, u, y in zip(ctxs, uu, ys):
+                c.a11(u, y, flags | ptq2.DEVICE)
+
+        ms_seq = time_applies(single, steps, warmup, stream)
+        ms_bat = time_applies(lambda: ptq2.a11_batch(ctxs, uu, ys, flags), steps, warmup, stream)
+        out[name] = {"sequential_ms": round(ms_seq, 4), "batched_ms": round(ms_bat, 4),
+                     "speedup": round(ms_seq / ms_bat, 3),
+                     "batched_gdofs": round(ndofs / (ms_bat * 1e-3) / 1e9, 3)}
+    for c in ctxs:
+        c.destroy()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,6 +387,10 @@ def main():
                     help="iteration cap of the solves (a profiled run uses a few dozen)")
     ap.add_argument("--solve-variants", default="fp64,fp32,mixed",
                     help="comma-separated subset of fp64, fp32, mixed")
+    ap.add_argument("--fused-smoother", type=int, default=None, metavar="0|1",
+                    help="Chebyshev steps with the update fused into the apply's store (default: the library's)")
+    ap.add_argument("--autotune", type=int, default=None, metavar="0|1|2",
+                    help="solver launch-shape autotuner (default: the library's, 1); 2 / 3 = row-split / MFMA kernel everywhere")
     ap.add_argument("--solve-only", action="store_true",
                     help="skip the apply timings and the CPU baseline: run the solve comparison only")
     ap.add_argument("--batch", action="store_true", help="also time 8 uneven solves, sequential vs concurrent")
@@ -344,6 +399,10 @@ def main():
                          "without it as 'solve', with it as 'solve_rigid')")
     ap.add_argument("--batch-rigid", choices=("off", "on", "both"), default="on",
                     help="rigid-body deflation in the batched solves (both: without it as 'batch_plain')")
+    ap.add_argument("--batch-apply", type=int, default=1, metavar="0|1",
+                    help="time the A11 of the 8 batched-solve problems: 8 single launches vs one batched launch")
+    ap.add_argument("--batch-apply-only", action="store_true",
+                    help="run only the batched-apply comparison (the command of its profiler runs)")
     ap.add_argument("--block", type=int, default=1, metavar="0|1",
                     help="time the full Stokes block (FP64 and FP32) on the contrast-1e6 sinker (default 1)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -356,7 +415,7 @@ def main():
     args = ap.parse_args()
     if args.apply_only:
         args.no_cpu = args.no_fp32 = True
-        args.solve = args.block = 0
+        args.solve = args.block = args.batch_apply = 0
         args.batch = False
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build runs on a single MI355X; --gpus %d is not supported"
@@ -378,13 +437,19 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": src.numel() * 8, "copy_probe_bytes_written": dst.numel() * 8}))
         return
 
+    if args.batch_apply_only:
+        extra = {"batch_apply": batch_apply_comparison(args.steps, args.warmup, torch.cuda.current_stream())}
+        print(json.dumps({"batch_apply_only": True, "extra": extra}), flush=True)
+        return
+
     if args.solve_only:
         rigid = {"off": [False], "on": [True], "both": [False, True]}[args.rigid]
         extra = {}
         for rg in rigid:
             key = "solve_rigid" if rg and args.rigid == "both" else "solve"
             extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rg, max_it=args.solve_max_it,
-                                          variants=tuple(args.solve_variants.split(",")))
+                                          variants=tuple(args.solve_variants.split(",")),
+                                          fused_smoother=args.fused_smoother, autotune=args.autotune)
         print(json.dumps({"solve_only": True, "extra": extra}), flush=True)
         return
 
@@ -438,12 +503,18 @@ def main():
             extra["body_force"] = body_force_timing(m, args.steps, args.warmup, stream)
         except Exception as exc:
             extra["body_force"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
+    if args.batch_apply:
+        try:
+            extra["batch_apply"] = batch_apply_comparison(args.steps, args.warmup, stream)
+        except Exception as exc:
+            extra["batch_apply"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     variants = {"off": [False], "on": [True], "both": [False, True]}
     for rigid in variants[args.rigid] if args.solve else []:
         key = "solve_rigid" if rigid and args.rigid == "both" else "solve"
         try:
             extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rigid, max_it=args.solve_max_it,
-                                          variants=tuple(args.solve_variants.split(",")))
+                                          variants=tuple(args.solve_variants.split(",")),
+                                          fused_smoother=args.fused_smoother, autotune=args.autotune)
         except Exception as exc:  # the apply line above stays valid; report the failure
             extra[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     for rigid in variants[args.batch_rigid] if args.batch else []:

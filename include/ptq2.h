@@ -52,6 +52,9 @@ enum {
 #define PTQ2_STOREDGEO 0x20u /* FP64 A11: read resident per-point geometric factors instead
                                (1944 B per element, built on the first such apply after a
                                mesh set-up); slower than recomputing on MI355X */
+#define PTQ2_MFMA 0x40u /* with PTQ2_FP32: the FP32 operator with the 81x27 reference gradient
+                           applied as a dense GEMM on the matrix cores (same factors; FP32
+                           rounding differs from the sum-factorized kernel's) */
 
 /* lifecycle ------------------------------------------------------------- */
 /* replaces MFA11CUDA_SetUp (include/stokes_operators.h:124-149) and the
@@ -101,6 +104,19 @@ int ptq2_a11_apply(ptq2_ctx *ctx, const void *u, void *y, unsigned flags);
    is PTQ2_ERR_SIZE.  Calls that overlap on one GPU (any stream, any context) race on that array;
    the library never allocates in this call.  Stream ordered, like the other PTQ2_DEVICE calls. */
 int ptq2_a11_apply_noalloc(ptq2_ctx *ctx, const double *u, double *y, unsigned flags);
+/* ptq2_a11_apply of n independent problems (1..PTQ2_BATCH_MAX contexts on one device, each
+   its own mesh and viscosity, meshes of any sizes) in ONE launch of the A11 kernel and one of
+   the face pass: y[i] = A11_i u[i].  The reference applies one operator per MatMult
+   (MatMult_MFStokes_A11, src/stokes_operators.c:1156-1222); the batch serves many small
+   sinker problems (the 24^3-80^3 batch) whose single launches would each fill a fraction of
+   the GPU.  flags must hold PTQ2_DEVICE and may add PTQ2_FP32 (float vectors, the smoother
+   operator), PTQ2_MATMULT or PTQ2_ADD; FP64 recomputes the geometry from the coordinates.
+   Results are bitwise those of ptq2_a11_apply on each problem.  The launch is ordered on
+   ctxs[0]'s stream after the work already queued on every ctxs[i]'s stream, and every
+   ctxs[i]'s stream waits for it.  Contexts and outputs must be distinct.  Device vectors are
+   not scanned for NaN/Inf. */
+#define PTQ2_BATCH_MAX 16
+int ptq2_a11_apply_batch(ptq2_ctx *const *ctxs, int n, const void *const *u, void *const *y, unsigned flags);
 /* diagonal of A11: MFStokesWrapper_diagA11 (src/stokes_operators_mf.c:71-135);
    with PTQ2_MATMULT, MatGetDiagonal_MFStokes_A11 (src/stokes_operators.c:1072-1113)
    which puts 1 in Dirichlet slots.  FP64 only. */

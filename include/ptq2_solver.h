@@ -60,6 +60,16 @@ typedef struct {
      updated there), plus a pass over the remaining lower-face nodes; results are bitwise those
      of 0 (default): the apply storing A x, then one update pass over all dofs */
   int fused_smoother;
+  /* 1 (default): at set-up, time each kernel of every level's smoother apply
+     (PTQ2_MG_KERNEL_*: one wave per brick of 4x4x4 elements; three waves per brick, each
+     running a slab of quadrature rows; with FP32 smoothing, the matrix-core form) with fast
+     or general tile addressing where every tile is full, and keep the fastest
+     (ptq2_mg_tuning reports the choice).  The kernels sum in different orders, so two
+     hierarchies whose choices differ round differently in the last bits.  0: one wave per
+     brick and the mesh's own tile addressing on every level.  2: the three-wave kernel on
+     every level, 3: the matrix-core form on every level (FP32 smoothing), untimed (for
+     tests and comparisons). */
+  int autotune;
 } ptq2_mg_opts;
 
 /* PTQ2_ETA_MEAN_Q1PROJ: the reference's coefficient hierarchy
@@ -106,6 +116,17 @@ int ptq2_mg_rigid_info(const ptq2_mg *mg, int *ninclusions, int *ncolumns, int32
 
 /* the Chebyshev interval [emin, emax] of every smoothed level (nlevels-1 entries each) */
 int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax);
+/* Kernels of the smoother apply (ptq2_mg_tuning) */
+enum { PTQ2_MG_KERNEL_BRICK = 0, PTQ2_MG_KERNEL_SPLIT = 1, PTQ2_MG_KERNEL_MFMA = 2 };
+/* Launch shape the autotuner chose for the smoother apply of each level l < nlevels - 1:
+   kernel (PTQ2_MG_KERNEL_*), tile addressing (1: fast full-tile path, 0: general), and the
+   measured ms per apply of the chosen and of the default shape (both 0 when autotune was off
+   or forced).  Any output may be NULL. */
+int ptq2_mg_tuning(const ptq2_mg *mg, int *kernel, int *fast_tiles, double *ms, double *ms_default);
+/* Per-level time of the V-cycle: reps V-cycles z = M^-1 r launched kernel by kernel (not from
+   the graph) with events between the levels; ms[l] = the mean time of level l's own work
+   (smoothing, residual and transfers; on the coarsest level its solve).  Synchronous. */
+int ptq2_mg_level_times(ptq2_mg *mg, const double *r, double *z, int reps, double *ms);
 
 /* order all subsequent solver work (every level and the captured V-cycle) on
    `hip_stream` (NULL = the null stream) */

@@ -245,6 +245,7 @@ int ptq2_destroy(ptq2_ctx **pc) {
   dev_free(c->fac32); dev_free(c->facb32); dev_free(c->geo64); dev_free(c->facebuf);
   dev_free(c->err); dev_free(c->su); dev_free(c->sy); dev_free(c->sp); dev_free(c->syp); dev_free(c->ures);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  if (c->ev) (void)hipEventDestroy(c->ev);
   delete c;
   *pc = nullptr;
   return PTQ2_OK;
@@ -408,8 +409,9 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
   int rc = check_ready(c, true);
   if (rc) return rc;
   if (!u || !y) return fail(c, PTQ2_ERR_INVALID_ARG, "null vector");
-  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY | PTQ2_STOREDGEO))
+  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD | PTQ2_ONTHEFLY | PTQ2_STOREDGEO | PTQ2_MFMA))
     return fail(c, PTQ2_ERR_INVALID_ARG, "bad flags");
+  if ((flags & PTQ2_MFMA) && !(flags & PTQ2_FP32)) return fail(c, PTQ2_ERR_INVALID_ARG, "PTQ2_MFMA needs PTQ2_FP32");
   if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
   if ((flags & PTQ2_ONTHEFLY) && (flags & PTQ2_STOREDGEO))
     return fail(c, PTQ2_ERR_INVALID_ARG, "ONTHEFLY and STOREDGEO exclusive");
@@ -435,8 +437,12 @@ int ptq2_a11_apply(ptq2_ctx *c, const void *u, void *y, unsigned flags) {
   void *tb = ptq2_internal_facebuf(c, &rc);
   if (rc) return rc;
   if (f32) {
-    HIPCHK(c, ptq2::launch_a11_fp32(c->box, (const float *)du, mask, c->fac32, (float *)dy, add, c->stream,
-                                    (float *)tb));
+    if (flags & PTQ2_MFMA)
+      HIPCHK(c, ptq2::launch_a11_fp32_mfma(c->box, (const float *)du, mask, c->fac32, (float *)dy, add, c->stream,
+                                           (float *)tb));
+    else
+      HIPCHK(c, ptq2::launch_a11_fp32(c->box, (const float *)du, mask, c->fac32, (float *)dy, add, c->stream,
+                                      (float *)tb));
     if (mm) HIPCHK(c, ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, (const float *)du, (float *)dy, 0, c->stream));
   } else {
     if (otf)
@@ -477,6 +483,90 @@ int ptq2_a11_apply_noalloc(ptq2_ctx *c, const double *u, double *y, unsigned fla
   HIPCHK(c, ptq2::launch_a11_fp64(c->box, c->coords, u, mm ? c->mask : nullptr, c->eta_bm, y, add, c->stream, scr));
   if (mm) HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, u, y, 0, c->stream));
   return PTQ2_OK;
+}
+
+}  // extern "C"
+
+namespace {
+int ctx_event(ptq2_ctx *c) {
+  if (!c->ev) HIPCHK(c, hipEventCreateWithFlags(&c->ev, hipEventDisableTiming));
+  return PTQ2_OK;
+}
+
+template <typename T>
+int a11_batch(ptq2_ctx *const *cs, int n, const void *const *u, void *const *y, unsigned flags) {
+  ptq2_ctx *c0 = cs[0];
+  const bool mm = flags & PTQ2_MATMULT, add = flags & PTQ2_ADD, f32 = sizeof(T) == 4;
+  int rc = set_device(c0);
+  if (rc) return rc;
+  ptq2::BatchArgs<T> a{};
+  a.n = n;
+  a.add = add ? 1 : 0;
+  int64_t blocks = 0;
+  for (int i = 0; i < n; i++) {
+    ptq2_ctx *c = cs[i];
+    rc = check_ready(c, true);
+    if (rc) return rc == PTQ2_ERR_INVALID_ARG ? fail(c0, rc, "null context %d", i) : fail(c0, rc, "problem %d: %s", i, c->last.c_str());
+    if (c->device != c0->device) return fail(c0, PTQ2_ERR_INVALID_ARG, "problem %d is on device %d, not %d", i, c->device, c0->device);
+    if (!u[i] || !y[i]) return fail(c0, PTQ2_ERR_INVALID_ARG, "null vector of problem %d", i);
+    for (int j = 0; j < i; j++)
+      if (y[j] == y[i] || cs[j] == c) return fail(c0, PTQ2_ERR_INVALID_ARG, "problems %d and %d share a context or an output", j, i);
+    if (f32 && (rc = build_fac32(c))) return fail(c0, rc, "problem %d: %s", i, c->last.c_str());
+    void *fb = ptq2_internal_facebuf(c, &rc);
+    if (rc) return fail(c0, rc, "problem %d: %s", i, c->last.c_str());
+    a.start[i] = (int)blocks;
+    a.box[i] = c->box;
+    a.u[i] = (const T *)u[i];
+    a.y[i] = (T *)y[i];
+    a.face[i] = (T *)fb;
+    a.mask[i] = mm ? c->mask : nullptr;
+    a.coords[i] = c->coords;
+    if constexpr (sizeof(T) == 4)
+      a.fac[i] = c->fac32;
+    else
+      a.fac[i] = c->eta_bm;
+    blocks += (c->nbricks + 7) / 8 * 8;
+    if (blocks > INT32_MAX) return fail(c0, PTQ2_ERR_SIZE, "batch of more than 2^31 bricks");
+  }
+  a.start[n] = (int)blocks;
+  // the launch runs on the first problem's stream, ordered after and before the work of every
+  // other problem's stream
+  if ((rc = ctx_event(c0))) return rc;
+  for (int i = 1; i < n; i++) {
+    ptq2_ctx *c = cs[i];
+    if (c->stream == c0->stream) continue;
+    if ((rc = ctx_event(c))) return fail(c0, rc, "problem %d: %s", i, c->last.c_str());
+    HIPCHK(c0, hipEventRecord(c->ev, c->stream));
+    HIPCHK(c0, hipStreamWaitEvent(c0->stream, c->ev, 0));
+  }
+  HIPCHK(c0, ptq2::launch_a11_batch<T>(a, c0->stream));
+  if (mm)
+    for (int i = 0; i < n; i++)
+      HIPCHK(c0, ptq2::launch_bc_rows<T>(cs[i]->ndir, cs[i]->dir_idx, (const T *)u[i], (T *)y[i], 0, c0->stream));
+  bool others = false;
+  for (int i = 1; i < n; i++) others |= cs[i]->stream != c0->stream;
+  if (others) {
+    HIPCHK(c0, hipEventRecord(c0->ev, c0->stream));
+    for (int i = 1; i < n; i++)
+      if (cs[i]->stream != c0->stream) HIPCHK(c0, hipStreamWaitEvent(cs[i]->stream, c0->ev, 0));
+  }
+  return PTQ2_OK;
+}
+}  // namespace
+
+extern "C" {
+
+static_assert(ptq2::BATCH_MAX == PTQ2_BATCH_MAX, "batch bound of the header and the kernels");
+
+int ptq2_a11_apply_batch(ptq2_ctx *const *ctxs, int n, const void *const *u, void *const *y, unsigned flags) {
+  if (!ctxs || !ctxs[0]) return PTQ2_ERR_INVALID_ARG;
+  ptq2_ctx *c0 = ctxs[0];
+  if (n < 1 || n > ptq2::BATCH_MAX) return fail(c0, PTQ2_ERR_INVALID_ARG, "batch of %d problems (1..%d)", n, ptq2::BATCH_MAX);
+  if (!u || !y) return fail(c0, PTQ2_ERR_INVALID_ARG, "null vector array");
+  if (!(flags & PTQ2_DEVICE)) return fail(c0, PTQ2_ERR_INVALID_ARG, "PTQ2_DEVICE is required");
+  if (flags & ~(PTQ2_DEVICE | PTQ2_FP32 | PTQ2_MATMULT | PTQ2_ADD)) return fail(c0, PTQ2_ERR_INVALID_ARG, "bad flags");
+  if ((flags & PTQ2_MATMULT) && (flags & PTQ2_ADD)) return fail(c0, PTQ2_ERR_INVALID_ARG, "MATMULT and ADD exclusive");
+  return (flags & PTQ2_FP32) ? a11_batch<float>(ctxs, n, u, y, flags) : a11_batch<double>(ctxs, n, u, y, flags);
 }
 
 int ptq2_a11_diag(ptq2_ctx *c, double *d, unsigned flags) {

@@ -41,6 +41,8 @@ struct ptq2_ctx {
   // u with the Dirichlet values inserted, for the residual (allocated on first use)
   double *ures = nullptr;
   size_t ures_bytes = 0;
+  // stream ordering of the batched apply (created on first use)
+  hipEvent_t ev = nullptr;
   // staging for host-pointer calls
   void *su = nullptr, *sy = nullptr;
   size_t su_bytes = 0, sy_bytes = 0;

@@ -432,10 +432,17 @@ struct ChebOut {
   const uint8_t *mask;
   T cr, ck, cx;
 };
+// The update from its operands.  Contraction is off: the compiler would otherwise fuse v + x into
+// an FMA at some call sites and not at others, and the two sites must agree bit for bit.
+template <typename T>
+__host__ __device__ __forceinline__ T cheb_value(const ChebOut<T> &co, T d, T bv, T y, T x, T pk, bool has_p) {
+#pragma clang fp contract(off)
+  const T v = (co.cr * d) * (bv - y);
+  return has_p ? fma(co.cx, x, fma(co.ck, pk, v)) : v + x;
+}
 template <typename T>
 __host__ __device__ __forceinline__ T cheb_update(const ChebOut<T> &co, int64_t i, T y) {
-  const T v = (co.cr * co.dinv[i]) * (co.b[i] - y);
-  return co.pkm1 ? fma(co.cx, co.x[i], fma(co.ck, co.pkm1[i], v)) : v + co.x[i];
+  return cheb_value(co, co.dinv[i], co.b[i], y, co.x[i], co.pkm1 ? co.pkm1[i] : T(0), co.pkm1 != nullptr);
 }
 
 // the full-tile case with the addressing of load_tile2_fast
@@ -449,6 +456,55 @@ __device__ __forceinline__ void store_tile_own_fast(const Box &b, const BrickId 
   const bool lasty = br.by == b.nby - 1, lastz = br.bz == b.nbz - 1;
   T *gb = g + 3 * t.node0;
   T *fo = face + (int64_t)br.lin * FACEVALS + L.c;
+  if constexpr (CH) {
+    // the update's operands of a chunk of steps are all loaded before the first is used (one
+    // exposed memory latency per chunk, not per step); every tile address is inside the box
+    constexpr int CHUNK = sizeof(T) == 4 ? TPAIRS : 14;
+    const T *bb = co->b + 3 * t.node0, *bd = co->dinv + 3 * t.node0, *bx = co->x + 3 * t.node0;
+    const T *bp = co->pkm1 ? co->pkm1 + 3 * t.node0 : nullptr;
+    const uint8_t *bm = co->mask ? co->mask + 3 * t.node0 : nullptr;
+    T *bo = co->out + 3 * t.node0;
+#pragma unroll
+    for (int c0 = 0; c0 < TPAIRS; c0 += CHUNK) {
+      T vb[CHUNK], vd[CHUNK], vx[CHUNK], vp[CHUNK];
+      int vm[CHUNK];
+#pragma unroll
+      for (int k = 0; k < CHUNK; k++) {
+        const int it = c0 + k;
+        if (it >= TPAIRS) break;
+        const uint32_t e = fast_eoff(b, L, it);
+        vb[k] = fast_at(bb, e);
+        vd[k] = fast_at(bd, e);
+        vx[k] = fast_at(bx, e);
+        vp[k] = bp ? fast_at(bp, e) : T(0);
+        vm[k] = bm ? (int)fast_at(bm, e) : 0;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int k = 0; k < CHUNK; k++) {
+        const int it = c0 + k;
+        if (it >= TPAIRS) break;
+        if (!fast_stores(L, it)) continue;
+        int y, z;
+        fast_row(L, it, y, z);
+        const bool owny = y < 2 * BE || lasty, ownz = z < 2 * BE || lastz;
+        const S val = s[(fast_wrap(it) ? L.ldsw : L.lds0) + fast_lds_step(it)];
+        const uint32_t e = fast_eoff(b, L, it);
+        if (ownx && owny && ownz) {
+          const bool lower = (L.ii == 0 && br.bx > 0) || (y == 0 && br.by > 0) || (z == 0 && br.bz > 0);
+          if (vm[k] || !lower) {
+            fast_at(bo, e) = cheb_value(*co, vd[k], vb[k], vm[k] ? vx[k] : T(val), vx[k], vp[k], bp != nullptr);
+          } else {
+            fast_at(gb, e) = T(val);
+          }
+        } else {
+          const int slot = !ownx ? z * TN + y : (!owny ? TN * TN + z * TN + L.ii : 2 * TN * TN + y * TN + L.ii);
+          fo[3 * slot] = T(val);
+        }
+      }
+    }
+    return;
+  }
   T v[TPAIRS];
   if (add_mode) {
 #pragma unroll
@@ -463,19 +519,7 @@ __device__ __forceinline__ void store_tile_own_fast(const Box &b, const BrickId 
     const bool owny = y < 2 * BE || lasty, ownz = z < 2 * BE || lastz;
     const S val = s[(fast_wrap(it) ? L.ldsw : L.lds0) + fast_lds_step(it)];
     if (ownx && owny && ownz) {
-      const uint32_t e = fast_eoff(b, L, it);
-      if constexpr (CH) {
-        // final unless a lower neighbour's face slot still adds to it (or Dirichlet: y = x)
-        const int64_t gi = 3 * t.node0 + e;
-        const bool dir = co->mask && co->mask[gi];
-        const bool lower = (L.ii == 0 && br.bx > 0) || (y == 0 && br.by > 0) || (z == 0 && br.bz > 0);
-        if (dir || !lower)
-          co->out[gi] = cheb_update(*co, gi, dir ? co->x[gi] : T(val));
-        else
-          fast_at(gb, e) = T(val);
-      } else {
-        fast_at(gb, e) = add_mode ? T(v[it] + val) : T(val);
-      }
+      fast_at(gb, fast_eoff(b, L, it)) = add_mode ? T(v[it] + val) : T(val);
     } else {
       const int slot = !ownx ? z * TN + y : (!owny ? TN * TN + z * TN + L.ii : 2 * TN * TN + y * TN + L.ii);
       fo[3 * slot] = T(val);
@@ -773,7 +817,9 @@ __device__ __forceinline__ void lds_copy_async(const void *src, void *lds, int l
   static_assert(NB % 16 == 0, "16-byte chunks");
   constexpr int CH = NB / 16, FULL = CH / 64, REM = CH % 64;
   const char *s0 = uniform_ptr((const char *)src);  // scalar base, per-lane byte offset
-  char *d0 = (char *)lds;
+  // the LDS destination goes to M0: a scalar even where it depends on the wave's index
+  typedef __attribute__((address_space(3))) char lchar;
+  lchar *d0 = (lchar *)(uintptr_t)__builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)(lchar *)lds);
   int off = 16 * lane;
   asm volatile("" : "+v"(off));  // one fresh vector register per copy site
 #pragma unroll

@@ -4,9 +4,28 @@
 #include <stdint.h>
 
 namespace ptq2 {
-struct Box;
+struct Box;  // defined in ptq2_device.h, which every includer includes first
 template <typename T>
 struct ChebOut;
+
+// Several problems (boxes) in one launch of the A11 kernel and one of the face pass.  Problem p
+// takes blocks start[p] .. start[p + 1] - 1 (start[p] a multiple of 8, at least its brick
+// count); fac is eta_bm for FP64 (geometry from coords) and the FP32 factors for FP32.
+constexpr int BATCH_MAX = 16;
+template <typename T>
+struct BatchArgs {
+  int n, add;
+  int start[BATCH_MAX + 1];
+  Box box[BATCH_MAX];
+  const T *u[BATCH_MAX];
+  T *y[BATCH_MAX];
+  T *face[BATCH_MAX];
+  const uint8_t *mask[BATCH_MAX];
+  const double *coords[BATCH_MAX];
+  const T *fac[BATCH_MAX];
+};
+template <typename T>
+hipError_t launch_a11_batch(const BatchArgs<T> &a, hipStream_t s);
 
 // error bits raised by validation kernels into a device int
 constexpr int ERRBIT_MESH = 1;
@@ -30,6 +49,20 @@ hipError_t launch_stokes_fp64(const Box &b, const double *coords, const double *
 hipError_t launch_a11_fp32(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y, bool add,
                            hipStream_t s, float *facebuf = nullptr, bool gather = true,
                            const ChebOut<float> *co = nullptr);
+// Row-split variants (3 waves per brick, one slab of quadrature rows each) for levels with few
+// bricks; one-launch output path only (facebuf required)
+hipError_t launch_a11_fp64_split(const Box &b, const double *coords, const double *u, const uint8_t *mask,
+                                 const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf,
+                                 bool gather = true);
+hipError_t launch_a11_fp32_split(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y,
+                                 bool add, hipStream_t s, float *facebuf, bool gather = true);
+// The FP32 smoother operator with the reference gradient as a dense GEMM on the matrix cores
+// (ptq2_mfma.hip); one-launch output path only
+hipError_t launch_a11_fp32_mfma(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y,
+                                bool add, hipStream_t s, float *facebuf, bool gather = true);
+// the face pass alone (adds the face slots of facebuf into their owners)
+hipError_t launch_gather_faces(const Box &b, const float *face, float *y, hipStream_t s);
+hipError_t launch_gather_faces(const Box &b, const double *face, double *y, hipStream_t s);
 hipError_t launch_build_fac32(const Box &b, const double *coords, const double *eta_bm, float *fac, float *facb,
                               int *err, hipStream_t s);
 hipError_t launch_a11_fp64g(const Box &b, const double *u, const uint8_t *mask, const double *geo, const double *eta_bm,

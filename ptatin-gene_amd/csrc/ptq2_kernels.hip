@@ -165,17 +165,17 @@ __device__ __forceinline__ void load3(const double *__restrict__ ep, int row, do
 // yp gets A21 u (and, with fp, minus the continuity source: FormFunctionLocal_P's
 // -int N^p fp, src/stokes_form_function.c:478-580).
 // CH: store with the fused Chebyshev update of co (store_tile_own<true>).
+// The brick br of box b (one wave); k_a11_fp64_t and the batched k_a11_fp64_batch resolve br.
 template <bool BLK, bool CH>
-__global__ __launch_bounds__(BEL) void k_a11_fp64_t(Box b, int color, const double *__restrict__ coords,
-                                                    const double *__restrict__ u, const uint8_t *__restrict__ mask,
-                                                    const double *__restrict__ eta_bm, double *__restrict__ y,
-                                                    int add_mode, double *__restrict__ facebuf,
-                                                    const double *__restrict__ p, double *__restrict__ yp,
-                                                    const double *__restrict__ fp, ChebOut<double> co) {
+__device__ __forceinline__ void a11_fp64_brick(const Box &b, const BrickId &br, int color,
+                                               const double *__restrict__ coords, const double *__restrict__ u,
+                                               const uint8_t *__restrict__ mask, const double *__restrict__ eta_bm,
+                                               double *__restrict__ y, int add_mode, double *__restrict__ facebuf,
+                                               const double *__restrict__ p, double *__restrict__ yp,
+                                               const double *__restrict__ fp, const ChebOut<double> &co) {
   __shared__ double Xs[3 * TCS];
   __shared__ double Us[3 * TCS];
   const int lane = threadIdx.x;
-  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
   double en[3];
@@ -236,6 +236,36 @@ __global__ __launch_bounds__(BEL) void k_a11_fp64_t(Box b, int color, const doub
     store_tile_own(b, br, t, Xs, y, facebuf, add_mode != 0, lane);
 }
 
+template <bool BLK, bool CH>
+__global__ __launch_bounds__(BEL) void k_a11_fp64_t(Box b, int color, const double *__restrict__ coords,
+                                                    const double *__restrict__ u, const uint8_t *__restrict__ mask,
+                                                    const double *__restrict__ eta_bm, double *__restrict__ y,
+                                                    int add_mode, double *__restrict__ facebuf,
+                                                    const double *__restrict__ p, double *__restrict__ yp,
+                                                    const double *__restrict__ fp, ChebOut<double> co) {
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
+  a11_fp64_brick<BLK, CH>(b, br, color, coords, u, mask, eta_bm, y, add_mode, facebuf, p, yp, fp, co);
+}
+
+// Several problems in one launch: block i belongs to the problem p with start[p] <= i <
+// start[p + 1] (starts are multiples of 8, so i and the local block index fall on the same
+// XCD and brick_xcd keeps its L2 grouping); the blocks past a problem's bricks return.
+__device__ __forceinline__ int batch_problem(const int *start, int n, int bid) {
+  int p = 0;
+#pragma unroll
+  for (int i = 1; i < BATCH_MAX; i++) p += (i < n && bid >= start[i]) ? 1 : 0;
+  return p;
+}
+
+__global__ __launch_bounds__(BEL) void k_a11_fp64_batch(BatchArgs<double> a) {
+  const int p = batch_problem(a.start, a.n, blockIdx.x);
+  const Box b = a.box[p];
+  const int local = (int)blockIdx.x - a.start[p];
+  if (local >= b.nbx * b.nby * b.nbz) return;
+  a11_fp64_brick<false, false>(b, brick_xcd(b, local), -1, a.coords[p], a.u[p], a.mask[p], a.fac[p], a.y[p], a.add,
+                               a.face[p], nullptr, nullptr, nullptr, ChebOut<double>{});
+}
+
 // ----------------------------------------------------------------- A11 FP32
 // Geometric factors are precomputed in FP64 (k_build_fac32) and stored rounded
 // to FP32 as F = sqrt(w eta / det) * adj(J), 9 per quadrature point, brick-major
@@ -292,21 +322,63 @@ __device__ __forceinline__ void slab_z32(const float *Us, int base, const Coef1 
   }
 }
 
+// FP32 factor row of one quadrature row for the 64 lanes: [qa*9+m][lane]
+constexpr int FROW = 27 * BEL;
+
+// Quadrature rows row0 .. row1-1 of the FP32 element, accumulated into Y.  On entry the factors
+// of row0 are on their way into Fs (lds_copy_async from fsrc + row0 * FROW); each row's
+// successor is copied in while it computes.  row0 starts a slab (row0 % 3 == 0).
+__device__ __forceinline__ void fp32_rows(const float *Us, float *Fs, const float *fsrc, int base, int lane, int row0,
+                                          int row1, float Y[3][3][3][3]) {
+  float ZB[3][3][3], ZD[3][3][3];
+#pragma nounroll
+  for (int row = row0; row < row1; row++) {
+    const int qb = row % 3, qc = row / 3;
+    const Coef1 cb = coef1(qb), cc = coef1(qc);
+    __builtin_amdgcn_s_waitcnt(WAIT_VM0);
+    float F0[9], F12[18];
+#pragma unroll
+    for (int n = 0; n < 9; n++) F0[n] = Fs[n * BEL + lane];
+#pragma unroll
+    for (int n = 0; n < 18; n++) F12[n] = Fs[(9 + n) * BEL + lane];
+    if (row + 1 < row1) {
+      __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
+      lds_copy_async<FROW * 4>(fsrc + (row + 1) * FROW, Fs, lane);
+    }
+    if (qb == 0) slab_z32(Us, base, cc, ZB, ZD);
+    float ur0[3][3], ur1[3][3], ur2[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        ur0[c][i] = fmac(cb.b[2], ZB[c][i][2], fmac(cb.b[1], ZB[c][i][1], static_cast<float>(cb.b[0]) * ZB[c][i][0]));
+        ur1[c][i] = fmac(cb.d[2], ZB[c][i][2], fmac(cb.d[1], ZB[c][i][1], static_cast<float>(cb.d[0]) * ZB[c][i][0]));
+        ur2[c][i] = fmac(cb.b[2], ZD[c][i][2], fmac(cb.b[1], ZD[c][i][1], static_cast<float>(cb.b[0]) * ZD[c][i][0]));
+      }
+    float P0[3][3], P1[3][3], P2[3][3];
+    zero33(P0); zero33(P1); zero33(P2);
+    a11_qp32<0>(ur0, ur1, ur2, F0, P0, P1, P2);
+    a11_qp32<1>(ur0, ur1, ur2, F12, P0, P1, P2);
+    a11_qp32<2>(ur0, ur1, ur2, F12 + 9, P0, P1, P2);
+    row_fence();
+    acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
+    row_fence();
+  }
+}
+
 template <bool CH>
-__global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
-                                                  const uint8_t *__restrict__ mask, const float *__restrict__ fac,
-                                                  float *__restrict__ y, int add_mode, float *__restrict__ facebuf,
-                                                  ChebOut<float> co) {
+__device__ __forceinline__ void a11_fp32_brick(const Box &b, const BrickId &br, int color, const float *__restrict__ u,
+                                               const uint8_t *__restrict__ mask, const float *__restrict__ fac,
+                                               float *__restrict__ y, int add_mode, float *__restrict__ facebuf,
+                                               const ChebOut<float> &co) {
   // LDS: the FP32 velocity tile and one quadrature row of factors ([qa*9+m][lane]) during
   // the rows, then the output tile in FP64 (LDS float atomics are many times slower than
   // ds_add_f64 on gfx950, and the wider sum costs nothing in the result)
   __shared__ double Ts[3 * TCS];
   float *Us = reinterpret_cast<float *>(Ts);
-  constexpr int FROW = 27 * BEL;
   float *Fs = Us + 3 * TCS;
   static_assert(sizeof(float) * (3 * TCS + FROW) <= sizeof(Ts), "velocity tile + factor row fit the output tile");
   const int lane = threadIdx.x;
-  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
   const float *fsrc = fac + (int64_t)br.lin * 243 * BEL;
   lds_copy_async<FROW * 4>(fsrc, Fs, lane);
@@ -326,42 +398,7 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
   // Row r+1's factors travel HBM -> LDS while row r computes.  Every lane runs the rows
   // (the copies need the whole wave); the inactive lanes of a partial brick compute on
   // tile and factor slots that no element owns and their outputs are discarded.
-  {
-    float ZB[3][3][3], ZD[3][3][3];
-#pragma nounroll
-    for (int row = 0; row < 9; row++) {
-      const int qb = row % 3, qc = row / 3;
-      const Coef1 cb = coef1(qb), cc = coef1(qc);
-      __builtin_amdgcn_s_waitcnt(WAIT_VM0);
-      float F0[9], F12[18];
-#pragma unroll
-      for (int n = 0; n < 9; n++) F0[n] = Fs[n * BEL + lane];
-#pragma unroll
-      for (int n = 0; n < 18; n++) F12[n] = Fs[(9 + n) * BEL + lane];
-      if (row < 8) {
-        __builtin_amdgcn_s_waitcnt(WAIT_LGKM0);
-        lds_copy_async<FROW * 4>(fsrc + (row + 1) * FROW, Fs, lane);
-      }
-      if (qb == 0) slab_z32(Us, base, cc, ZB, ZD);
-      float ur0[3][3], ur1[3][3], ur2[3][3];
-#pragma unroll
-      for (int c = 0; c < 3; c++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          ur0[c][i] = fmac(cb.b[2], ZB[c][i][2], fmac(cb.b[1], ZB[c][i][1], static_cast<float>(cb.b[0]) * ZB[c][i][0]));
-          ur1[c][i] = fmac(cb.d[2], ZB[c][i][2], fmac(cb.d[1], ZB[c][i][1], static_cast<float>(cb.d[0]) * ZB[c][i][0]));
-          ur2[c][i] = fmac(cb.b[2], ZD[c][i][2], fmac(cb.b[1], ZD[c][i][1], static_cast<float>(cb.b[0]) * ZD[c][i][0]));
-        }
-      float P0[3][3], P1[3][3], P2[3][3];
-      zero33(P0); zero33(P1); zero33(P2);
-      a11_qp32<0>(ur0, ur1, ur2, F0, P0, P1, P2);
-      a11_qp32<1>(ur0, ur1, ur2, F12, P0, P1, P2);
-      a11_qp32<2>(ur0, ur1, ur2, F12 + 9, P0, P1, P2);
-      row_fence();
-      acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
-      row_fence();
-    }
-  }
+  fp32_rows(Us, Fs, fsrc, base, lane, 0, 9, Y);
   __syncthreads();
   zero_tile(Ts, lane);
   __syncthreads();
@@ -373,6 +410,116 @@ __global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float 
     store_tile_own<true>(b, br, t, Ts, y, facebuf, false, lane, &co);
   else
     store_tile_own(b, br, t, Ts, y, facebuf, add_mode != 0, lane);
+}
+
+template <bool CH>
+__global__ __launch_bounds__(BEL) void k_a11_fp32(Box b, int color, const float *__restrict__ u,
+                                                  const uint8_t *__restrict__ mask, const float *__restrict__ fac,
+                                                  float *__restrict__ y, int add_mode, float *__restrict__ facebuf,
+                                                  ChebOut<float> co) {
+  const BrickId br = color >= 0 ? brick_of(b, color, blockIdx.x) : brick_xcd(b, blockIdx.x);
+  a11_fp32_brick<CH>(b, br, color, u, mask, fac, y, add_mode, facebuf, co);
+}
+
+__global__ __launch_bounds__(BEL) void k_a11_fp32_batch(BatchArgs<float> a) {
+  const int p = batch_problem(a.start, a.n, blockIdx.x);
+  const Box b = a.box[p];
+  const int local = (int)blockIdx.x - a.start[p];
+  if (local >= b.nbx * b.nby * b.nbz) return;
+  a11_fp32_brick<false>(b, brick_xcd(b, local), -1, a.u[p], a.mask[p], a.fac[p], a.y[p], a.add, a.face[p],
+                        ChebOut<float>{});
+}
+
+// ------------------------------------------------ row-split variants (small levels)
+// Three waves per brick: wave w runs the slab of quadrature rows 3w .. 3w+2 (qc = w) into its own
+// partial element outputs, and the partials go into the output tile wave after wave (w = 0, 1,
+// 2), so the result does not depend on the waves' timing.  A brick's rows take a third of the
+// time, which pays where a level has too few bricks to fill the GPU; the sum is taken in
+// another order than the one-wave kernels' (rounding differs in the last bits).  One-launch
+// output path only (facebuf).
+__device__ __forceinline__ int split_wave() { return __builtin_amdgcn_readfirstlane((int)threadIdx.x / BEL); }
+
+__global__ __launch_bounds__(3 * BEL) void k_a11_fp64_split(Box b, const double *__restrict__ coords,
+                                                            const double *__restrict__ u,
+                                                            const uint8_t *__restrict__ mask,
+                                                            const double *__restrict__ eta_bm, double *__restrict__ y,
+                                                            int add_mode, double *__restrict__ facebuf) {
+  __shared__ double Xs[3 * TCS];
+  __shared__ double Us[3 * TCS];
+  const int w = split_wave(), lane = (int)threadIdx.x - BEL * w;
+  const BrickId br = brick_xcd(b, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  const double *ep = eta_bm + (int64_t)br.lin * 27 * BEL + lane;
+  double en[3];
+  load3(ep, 3 * w, en);
+  if (w == 0) load_tile2(b, t, br, coords, Xs, u, mask, Us, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  double Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
+  BlockState bs;
+  if (active) {
+    double ca[9], cb9[9];
+    ld_col(Xs, base, 0, ca);
+#pragma nounroll
+    for (int row = 3 * w; row < 3 * w + 3; row++) {
+      const int qb = row % 3, qc = row / 3;
+      double e3[3] = {en[0], en[1], en[2]};
+      if (row + 1 < 3 * w + 3) load3(ep, row + 1, en);
+      a11_row64<false>(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y, row, &bs);
+    }
+  }
+  __syncthreads();
+  if (w == 0) zero_tile(Xs, lane);
+  __syncthreads();
+  for (int k = 0; k < 3; k++) {
+    if (w == k && active) emit_element(Xs, base, Y);
+    __syncthreads();
+  }
+  if (w == 0) store_tile_own(b, br, t, Xs, y, facebuf, add_mode != 0, lane);
+}
+
+__global__ __launch_bounds__(3 * BEL) void k_a11_fp32_split(Box b, const float *__restrict__ u,
+                                                            const uint8_t *__restrict__ mask,
+                                                            const float *__restrict__ fac, float *__restrict__ y,
+                                                            int add_mode, float *__restrict__ facebuf) {
+  __shared__ double Ts[3 * TCS];
+  __shared__ float Us[3 * TCS];
+  __shared__ float Fs[3][FROW];
+  const int w = split_wave(), lane = (int)threadIdx.x - BEL * w;
+  const BrickId br = brick_xcd(b, blockIdx.x);
+  const Tile t = tile_of(b, br);
+  const float *fsrc = fac + (int64_t)br.lin * 243 * BEL;
+  lds_copy_async<FROW * 4>(fsrc + 3 * w * FROW, Fs[w], lane);
+  if (w == 0) load_tile(b, t, br, u, mask, Us, lane);
+  if (w == 1) zero_tile(Ts, lane);
+  __syncthreads();
+  const int base = lane_base(lane);
+  const bool active = lane_active(t, lane);
+  float Y[3][3][3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0f;
+  fp32_rows(Us, Fs[w], fsrc, base, lane, 3 * w, 3 * w + 3, Y);
+  __syncthreads();
+  for (int k = 0; k < 3; k++) {
+    if (w == k && active) emit_element(Ts, base, Y);
+    __syncthreads();
+  }
+  if (w == 0) store_tile_own(b, br, t, Ts, y, facebuf, add_mode != 0, lane);
 }
 
 // ------------------------------------------------------ A11 FP64, stored geometry
@@ -1107,12 +1254,12 @@ static inline int nbricks(const Box &b) { return b.nbx * b.nby * b.nbz; }
 // same L2 as their writer); then each thread takes up to 3 of the brick's at most 3 x 3 x TN x TN
 // face values.
 template <typename T>
-__global__ __launch_bounds__(256) void k_gather_faces(Box b, const T *__restrict__ face, T *__restrict__ y) {
+__device__ __forceinline__ void gather_faces_brick(const Box &b, const BrickId &br, const T *__restrict__ face,
+                                                   T *__restrict__ y) {
   constexpr int F2 = TN * TN, UP = 2 * BE;
   constexpr int OFX = 0, OFY = 3 * F2, OFZ = 6 * F2, OXY = 9 * F2, OXZ = OXY + 3 * TN, OYZ = OXZ + 3 * TN,
                 OC = OYZ + 3 * TN, NS = OC + 3;
   __shared__ T sh[NS];
-  const BrickId br = brick_xcd(b, blockIdx.x);
   const bool hx = br.bx > 0, hy = br.by > 0, hz = br.bz > 0;
   if (!(hx || hy || hz)) return;
   const Tile t = tile_of(b, br);
@@ -1187,6 +1334,20 @@ __global__ __launch_bounds__(256) void k_gather_faces(Box b, const T *__restrict
 }
 
 template <typename T>
+__global__ __launch_bounds__(256) void k_gather_faces(Box b, const T *__restrict__ face, T *__restrict__ y) {
+  gather_faces_brick(b, brick_xcd(b, blockIdx.x), face, y);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_faces_batch(BatchArgs<T> a) {
+  const int p = batch_problem(a.start, a.n, blockIdx.x);
+  const Box b = a.box[p];
+  const int local = (int)blockIdx.x - a.start[p];
+  if (local >= b.nbx * b.nby * b.nbz) return;
+  gather_faces_brick(b, brick_xcd(b, local), (const T *)a.face[p], a.y[p]);
+}
+
+template <typename T>
 hipError_t launch_gather(const Box &b, const T *face, T *y, hipStream_t s) {
   hipLaunchKernelGGL(k_gather_faces<T>, dim3(nbricks(b)), dim3(256), 0, s, b, face, y);
   return hipGetLastError();
@@ -1215,6 +1376,45 @@ hipError_t launch_a11_fp64(const Box &b, const double *coords, const double *u, 
   }
   return hipGetLastError();
 }
+
+hipError_t launch_gather_faces(const Box &b, const float *face, float *y, hipStream_t s) {
+  return launch_gather(b, face, y, s);
+}
+hipError_t launch_gather_faces(const Box &b, const double *face, double *y, hipStream_t s) {
+  return launch_gather(b, face, y, s);
+}
+
+hipError_t launch_a11_fp64_split(const Box &b, const double *coords, const double *u, const uint8_t *mask,
+                                 const double *eta_bm, double *y, bool add, hipStream_t s, double *facebuf,
+                                 bool gather) {
+  if (!facebuf) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_a11_fp64_split, dim3(nbricks(b)), dim3(3 * BEL), 0, s, b, coords, u, mask, eta_bm, y,
+                     add ? 1 : 0, facebuf);
+  return gather ? launch_gather(b, (const double *)facebuf, y, s) : hipGetLastError();
+}
+
+hipError_t launch_a11_fp32_split(const Box &b, const float *u, const uint8_t *mask, const float *fac, float *y,
+                                 bool add, hipStream_t s, float *facebuf, bool gather) {
+  if (!facebuf) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_a11_fp32_split, dim3(nbricks(b)), dim3(3 * BEL), 0, s, b, u, mask, fac, y, add ? 1 : 0,
+                     facebuf);
+  return gather ? launch_gather(b, (const float *)facebuf, y, s) : hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_a11_batch(const BatchArgs<T> &a, hipStream_t s) {
+  const int nb = a.start[a.n];
+  if constexpr (sizeof(T) == 8)
+    hipLaunchKernelGGL(k_a11_fp64_batch, dim3(nb), dim3(BEL), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_a11_fp32_batch, dim3(nb), dim3(BEL), 0, s, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_gather_faces_batch<T>, dim3(nb), dim3(256), 0, s, a);
+  return hipGetLastError();
+}
+template hipError_t launch_a11_batch<double>(const BatchArgs<double> &, hipStream_t);
+template hipError_t launch_a11_batch<float>(const BatchArgs<float> &, hipStream_t);
 
 hipError_t launch_stokes_fp64(const Box &b, const double *coords, const double *u, const double *p,
                               const uint8_t *mask, const double *eta_bm, double *yu, double *yp, const double *fp,

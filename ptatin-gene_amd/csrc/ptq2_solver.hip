@@ -959,8 +959,20 @@ struct DevBuf {
 
 }  // namespace
 
+// launch shape of a level's smoother apply, picked at set-up by timing (autotune)
+struct LevelTune {
+  int split = 0;              // kernel: 0 one wave per brick, 1 row-split (3 waves per brick),
+                              // 2 matrix-core form (FP32 smoothing only)
+  int fast = 0;               // tile addressing of full tiles: 1 = 32-bit fast path, 0 = general
+  double ms = 0.0, ms_default = 0.0;  // measured time per apply: chosen shape, default shape
+};
+
 struct ptq2_mg {
   ptq2_mg_opts opt{};
+  std::vector<LevelTune> tune;          // per level (smoothing levels)
+  std::vector<ptq2::Box> lbox;          // per level: the box of the level's launches (tuned fast32)
+  bool prof = false;                    // record per-level events in the eager V-cycle
+  std::vector<hipEvent_t> pev;          // 4 per level: entry, before / after the coarser level, exit
   int nl = 0;
   bool f32 = true;
   hipStream_t stream = nullptr;
@@ -1027,39 +1039,49 @@ int mg_alloc(ptq2_mg *mg, void **p, size_t bytes) {
 // y = A x on level l in precision T (MatMult semantics: masked columns, y_D = x_D)
 template <typename T>
 int level_matmult(ptq2_mg *mg, int l, const T *x, T *y);
-template <>
-int level_matmult<double>(ptq2_mg *mg, int l, const double *x, double *y) {
+// the level's A11 apply in its tuned launch shape (gather: with the face pass)
+// (the row-split choice is made for the smoothing precision and applies to it only)
+hipError_t level_launch(ptq2_mg *mg, int l, const double *x, double *y, bool gather) {
   ptq2_ctx *c = mg->ctx[l];
+  const ptq2::Box &b = mg->lbox[l];
   // geometry recomputed from the coordinates (the faster FP64 kernel); the level's face
   // buffer was allocated by its set-up applies (no allocation here: this runs under graph
   // capture)
-  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream,
-                              (double *)c->facebuf));
+  if (mg->tune[l].split == 1 && !mg->f32)
+    return ptq2::launch_a11_fp64_split(b, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream,
+                                       (double *)c->facebuf, gather);
+  return ptq2::launch_a11_fp64(b, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream, (double *)c->facebuf,
+                               gather);
+}
+hipError_t level_launch(ptq2_mg *mg, int l, const float *x, float *y, bool gather) {
+  ptq2_ctx *c = mg->ctx[l];
+  const ptq2::Box &b = mg->lbox[l];
+  if (mg->tune[l].split == 1 && mg->f32)
+    return ptq2::launch_a11_fp32_split(b, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, gather);
+  if (mg->tune[l].split == 2 && mg->f32)
+    return ptq2::launch_a11_fp32_mfma(b, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, gather);
+  return ptq2::launch_a11_fp32(b, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, gather);
+}
+
+template <>
+int level_matmult<double>(ptq2_mg *mg, int l, const double *x, double *y) {
+  ptq2_ctx *c = mg->ctx[l];
+  MGCHK(level_launch(mg, l, x, y, true));
   MGCHK(ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
   return PTQ2_OK;
 }
 template <>
 int level_matmult<float>(ptq2_mg *mg, int l, const float *x, float *y) {
   ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf));
+  MGCHK(level_launch(mg, l, x, y, true));
   MGCHK(ptq2::launch_bc_rows<float>(c->ndir, c->dir_idx, x, y, 0, mg->stream));
   return PTQ2_OK;
 }
 
 // the level's A11 apply without its face pass and Dirichlet rows: k_cheb_faces completes y
 template <typename T>
-int level_apply_owned(ptq2_mg *mg, int l, const T *x, T *y);
-template <>
-int level_apply_owned<double>(ptq2_mg *mg, int l, const double *x, double *y) {
-  ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream, (double *)c->facebuf,
-                              false));
-  return PTQ2_OK;
-}
-template <>
-int level_apply_owned<float>(ptq2_mg *mg, int l, const float *x, float *y) {
-  ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, false));
+int level_apply_owned(ptq2_mg *mg, int l, const T *x, T *y) {
+  MGCHK(level_launch(mg, l, x, y, false));
   return PTQ2_OK;
 }
 
@@ -1070,14 +1092,15 @@ int level_apply_cheb(ptq2_mg *mg, int l, const T *x, T *y, const ptq2::ChebOut<T
 template <>
 int level_apply_cheb<double>(ptq2_mg *mg, int l, const double *x, double *y, const ptq2::ChebOut<double> &co) {
   ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp64(c->box, c->coords, x, c->mask, c->eta_bm, y, false, mg->stream, (double *)c->facebuf,
+  MGCHK(ptq2::launch_a11_fp64(mg->lbox[l], c->coords, x, c->mask, c->eta_bm, y, false, mg->stream, (double *)c->facebuf,
                               false, &co));
   return PTQ2_OK;
 }
 template <>
 int level_apply_cheb<float>(ptq2_mg *mg, int l, const float *x, float *y, const ptq2::ChebOut<float> &co) {
   ptq2_ctx *c = mg->ctx[l];
-  MGCHK(ptq2::launch_a11_fp32(c->box, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, false, &co));
+  MGCHK(ptq2::launch_a11_fp32(mg->lbox[l], x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, false,
+                              &co));
   return PTQ2_OK;
 }
 
@@ -1100,7 +1123,7 @@ int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
   // one Chebyshev step from xin: the apply, then the update.  Fused: the apply's store
   // updates the final owned dofs and the face pass the lower-face ones; otherwise the apply
   // stores A xin and the face pass updates every dof.
-  const bool fz = mg->opt.fused_smoother != 0 && lc->facebuf;
+  const bool fz = mg->opt.fused_smoother != 0 && lc->facebuf && mg->tune[l].split == 0;
   auto step = [&](const T *xin, const T *pkm1, T *out, double cr, double ck, double cx) -> int {
     const ptq2::ChebOut<T> co{(const T *)b, (const T *)dinv, xin, pkm1, out, (const uint8_t *)lc->mask, (T)cr, (T)ck,
                               (T)cx};
@@ -1133,14 +1156,21 @@ int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
   return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "chebyshev launch");
 }
 
+// per-level timing (eager V-cycles only): event k of level l
+void prof_mark(ptq2_mg *mg, int l, int k) {
+  if (mg->prof) (void)hipEventRecord(mg->pev[4 * l + k], mg->stream);
+}
+
 template <typename T>
 int vcycle_level(ptq2_mg *mg, int l) {
   const int64_t n = mg->n[l];
   T *b = (T *)mg->vb[l], *x = (T *)mg->vx[l], *y = (T *)mg->vy[l];
+  prof_mark(mg, l, 0);
   if (l == mg->nl - 1) {
     int nn = (int)n;
     hipLaunchKernelGGL(k_dense_gemv<T>, dim3((nn + 3) / 4), dim3(256), 0, mg->stream, nn, mg->ainv_ld, (const double *)mg->ainv,
                        (const T *)b, x);
+    prof_mark(mg, l, 3);
     return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "coarse solve launch");
   }
   MGRC(chebyshev<T>(mg, l, true));
@@ -1150,11 +1180,90 @@ int vcycle_level(ptq2_mg *mg, int l) {
   const int NXc = 2 * mc[0] + 1, NYc = 2 * mc[1] + 1, NZc = 2 * mc[2] + 1;
   hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(mg->n[l + 1])), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, NXf, NYf, NZf,
                      (const T *)b, (const T *)y, (const uint8_t *)mg->ctx[l + 1]->mask, (T *)mg->vb[l + 1]);
+  prof_mark(mg, l, 1);
   MGRC(vcycle_level<T>(mg, l + 1));
+  prof_mark(mg, l, 2);
   hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, NXc, NYc, NXf, NYf, NZf,
                      (const T *)mg->vx[l + 1], (const uint8_t *)mg->ctx[l]->mask, x);
   MGRC(chebyshev<T>(mg, l, false));
+  prof_mark(mg, l, 3);
   return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle launch");
+}
+
+// ms per apply of level l's smoother apply (no face pass) in the current launch shape: the
+// fastest of 3 timed batches of reps launches, after 2 untimed ones
+template <typename T>
+int time_level(ptq2_mg *mg, int l, int reps, double *ms) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  MGCHK(hipEventCreate(&e0));
+  hipError_t e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return mgfail(mg, PTQ2_ERR_DEVICE, hipGetErrorString(e));
+  }
+  const T *x = (const T *)mg->vx[l];
+  T *y = (T *)mg->vy[l];
+  double best = 1e30;
+  for (int k = 0; k < 2 && e == hipSuccess; k++) e = level_launch(mg, l, x, y, false);
+  for (int k = 0; k < 3 && e == hipSuccess; k++) {
+    e = hipEventRecord(e0, mg->stream);
+    for (int i = 0; i < reps && e == hipSuccess; i++) e = level_launch(mg, l, x, y, false);
+    if (e == hipSuccess) e = hipEventRecord(e1, mg->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    if (e == hipSuccess) best = std::min(best, (double)t / reps);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (e != hipSuccess) return mgfail(mg, PTQ2_ERR_DEVICE, hipGetErrorString(e));
+  *ms = best;
+  return PTQ2_OK;
+}
+
+// Autotuner of the smoother apply, per level: times each launch shape (one wave per brick, the
+// row-split kernel's three, and for FP32 smoothing the matrix-core form; fast or general tile
+// addressing where the tiles are full) and keeps the fastest, the default shape unless another
+// is at least 3% faster.  Levels with few bricks (e.g. 16^3: 64 bricks for 1,024 SIMDs) are
+// latency-bound in the one-wave kernel.
+template <typename T>
+int autotune(ptq2_mg *mg) {
+  for (int l = 0; l + 1 < mg->nl; l++) {
+    ptq2_ctx *c = mg->ctx[l];
+    LevelTune &t = mg->tune[l];
+    if (!mg->opt.autotune || !c->facebuf) continue;
+    if (mg->opt.autotune >= 2) {  // one kernel everywhere, untimed
+      t.split = mg->opt.autotune == 2 ? 1 : (sizeof(T) == 4 ? 2 : 0);
+      continue;
+    }
+    const int64_t n = mg->n[l];
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (T *)mg->vx[l], T(1));
+    double t0 = 0.0;
+    MGRC(time_level<T>(mg, l, 2, &t0));
+    const int reps = (int)std::min(50.0, std::max(2.0, 2.0 / std::max(t0, 1e-3)));  // ~2 ms per batch
+    double best = 1e30;
+    int bs = 0, bf = c->box.fast32;
+    const int kinds = sizeof(T) == 4 ? 3 : 2;
+    for (int split = 0; split < kinds; split++)
+      for (int fast = c->box.fast32; fast >= 0; fast--) {
+        t.split = split;
+        mg->lbox[l].fast32 = fast;
+        double ms = 0.0;
+        MGRC(time_level<T>(mg, l, reps, &ms));
+        const bool dflt = split == 0 && fast == c->box.fast32;
+        if (dflt) t.ms_default = ms;
+        if (dflt ? ms < best / 0.97 : ms < 0.97 * best) {
+          best = ms;
+          bs = split;
+          bf = fast;
+        }
+      }
+    t.split = bs;
+    t.fast = bf;
+    t.ms = best;
+    mg->lbox[l].fast32 = bf;
+  }
+  return PTQ2_OK;
 }
 
 // A-DEF2 steps around the V-cycle on the level-0 vectors: before it b -= A W y with
@@ -1739,12 +1848,15 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->rigid_min_contrast = 100.0;
   o->rigid_max_inclusions = RG_MAXINC;
   o->fused_smoother = 0;
+  o->autotune = 1;
 }
 
 int ptq2_mg_destroy(ptq2_mg **pmg) {
   if (!pmg || !*pmg) return PTQ2_OK;
   ptq2_mg *mg = *pmg;
   if (mg->stream) (void)hipStreamSynchronize(mg->stream);
+  for (hipEvent_t e : mg->pev)
+    if (e) (void)hipEventDestroy(e);
   if (mg->exec) (void)hipGraphExecDestroy(mg->exec);
   if (mg->graph) (void)hipGraphDestroy(mg->graph);
   for (size_t l = 0; l < mg->vb.size(); l++) {
@@ -1778,6 +1890,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   if (opts) o = *opts;
   else ptq2_mg_default_opts(&o);
   if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0) ||
+      o.autotune < 0 || o.autotune > 3 ||
       o.coarse_eta_mean < PTQ2_ETA_MEAN_ARITHMETIC || o.coarse_eta_mean > PTQ2_ETA_MEAN_Q1PROJ)
     return PTQ2_ERR_INVALID_ARG;
   if (o.rigid_deflation &&
@@ -1873,6 +1986,11 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     mf.swap(mc);
   }
   if (mg->n.back() > o.coarse_max_dofs) return mgfail(mg, PTQ2_ERR_SIZE, "coarse level too large for the dense solve");
+  mg->tune.assign(mg->nl, LevelTune{});
+  for (int l = 0; l < mg->nl; l++) {
+    mg->lbox.push_back(mg->ctx[l]->box);
+    mg->tune[l].fast = mg->ctx[l]->box.fast32;
+  }
 
   // ---- small reduction buffers (also used by the solve)
   MGRC(mg_alloc(mg, (void **)&mg->part, sizeof(double) * (2 * MAXRESTART + 4) * MDB));
@@ -1949,6 +2067,13 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     (void)hipFree(d); (void)hipFree(v); (void)hipFree(w);
     MGRC(rc);
   }
+  // ---- launch shapes of the smoother applies
+  for (int l = 0; l + 1 < mg->nl; l++) {
+    int rc = PTQ2_OK;
+    (void)ptq2_internal_facebuf(mg->ctx[l], &rc);
+    MGRC(rc);
+  }
+  MGRC(mg->f32 ? autotune<float>(mg) : autotune<double>(mg));
   // ---- dense coarse inverse, assembled and inverted on the device
   {
     const int l = mg->nl - 1;
@@ -2091,6 +2216,17 @@ int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax) {
   return PTQ2_OK;
 }
 
+int ptq2_mg_tuning(const ptq2_mg *mg, int *kernel, int *fast_tiles, double *ms, double *ms_default) {
+  if (!mg) return PTQ2_ERR_INVALID_ARG;
+  for (int l = 0; l + 1 < mg->nl; l++) {
+    if (kernel) kernel[l] = mg->tune[l].split;
+    if (fast_tiles) fast_tiles[l] = mg->tune[l].fast;
+    if (ms) ms[l] = mg->tune[l].ms;
+    if (ms_default) ms_default[l] = mg->tune[l].ms_default;
+  }
+  return PTQ2_OK;
+}
+
 int ptq2_mg_set_stream(ptq2_mg *mg, void *s) {
   if (!mg) return PTQ2_ERR_INVALID_ARG;
   mg->stream = (hipStream_t)s;
@@ -2113,6 +2249,38 @@ int ptq2_mg_vcycle(ptq2_mg *mg, const double *r, double *z) {
   if (!mg || !r || !z) return PTQ2_ERR_INVALID_ARG;
   MGRC(check_fine(mg));
   return vcycle_scaled(mg, r, 1.0, z);
+}
+
+int ptq2_mg_level_times(ptq2_mg *mg, const double *r, double *z, int reps, double *ms) {
+  if (!mg || !r || !z || !ms || reps < 1) return PTQ2_ERR_INVALID_ARG;
+  MGRC(check_fine(mg));
+  if (mg->pev.empty()) {
+    mg->pev.assign(4 * (size_t)mg->nl, nullptr);
+    for (hipEvent_t &e : mg->pev) MGCHK(hipEventCreate(&e));
+  }
+  std::vector<double> acc((size_t)mg->nl, 0.0);
+  hipLaunchKernelGGL(k_set_io, dim3(1), dim3(64), 0, mg->stream, mg->io, r, z, 1.0, 1.0);
+  MGCHK(hipGetLastError());
+  for (int it = 0; it < reps; it++) {
+    mg->prof = true;
+    const int rc = vcycle_body(mg);
+    mg->prof = false;
+    MGRC(rc);
+    MGCHK(hipStreamSynchronize(mg->stream));
+    for (int l = 0; l < mg->nl; l++) {
+      float a = 0.0f, b = 0.0f;
+      hipEvent_t *e = &mg->pev[4 * (size_t)l];
+      if (l == mg->nl - 1) {
+        MGCHK(hipEventElapsedTime(&a, e[0], e[3]));
+      } else {
+        MGCHK(hipEventElapsedTime(&a, e[0], e[1]));
+        MGCHK(hipEventElapsedTime(&b, e[2], e[3]));
+      }
+      acc[l] += (double)a + (double)b;
+    }
+  }
+  for (int l = 0; l < mg->nl; l++) ms[l] = acc[l] / reps;
+  return PTQ2_OK;
 }
 
 int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_it, int restart, int *iters,

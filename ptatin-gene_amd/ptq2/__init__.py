@@ -6,4 +6,4 @@ objects and of the solver that consumes them.
 """
 from . import mesh  # noqa: F401
 from ._lib import (ADD, DEVICE, FP32, MATMULT, PTQ2Error, LIB_PATH, SIGNATURES)  # noqa: F401
-from .operators import Context, MatA11MF, MatStokesMF, register_a11_op  # noqa: F401
+from .operators import Context, MatA11MF, MatStokesMF, a11_batch, register_a11_op  # noqa: F401

@@ -26,6 +26,7 @@ MATMULT = 0x4
 ADD = 0x8
 ONTHEFLY = 0x10
 STOREDGEO = 0x20
+MFMA = 0x40
 
 # every entry point of include/ptq2.h with its ctypes signature
 _c_int, _c_uint, _vp, _i64 = ctypes.c_int, ctypes.c_uint, ctypes.c_void_p, ctypes.c_int64
@@ -43,6 +44,7 @@ SIGNATURES = {
     "ptq2_set_viscosity": (_c_int, [_vp, _vp, ctypes.c_uint64, _c_uint]),
     "ptq2_a11_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_a11_apply_noalloc": (_c_int, [_vp, _vp, _vp, _c_uint]),
+    "ptq2_a11_apply_batch": (_c_int, [_vp, _c_int, _vp, _vp, _c_uint]),
     "ptq2_a11_diag": (_c_int, [_vp, _vp, _c_uint]),
     "ptq2_a12_apply": (_c_int, [_vp, _vp, _vp, _c_uint]),
     "ptq2_body_force": (_c_int, [_vp, _vp, _vp, _c_uint]),

@@ -232,6 +232,39 @@ class Context:
         return yu, yp
 
 
+BATCH_MAX = 16  # PTQ2_BATCH_MAX
+
+
+def a11_batch(ctxs, us, ys=None, flags=0):
+    """A11 of several problems (contexts on one GPU, meshes of any sizes) in one launch of the
+    apply kernel and one of the face pass (ptq2_a11_apply_batch): ys[i] = A11_i us[i].  Device
+    tensors only (float64, or float32 with FP32 in flags); ordered on torch's current stream."""
+    n = len(ctxs)
+    if not 1 <= n <= BATCH_MAX or len(us) != n:
+        raise ValueError("a11_batch takes 1..%d problems and one input each" % BATCH_MAX)
+    if not all(_is_dev(u) for u in us):
+        raise ValueError("a11_batch takes device tensors")
+    dt = torch.float32 if flags & FP32 else torch.float64
+    for c, u in zip(ctxs, us):
+        if u.dtype != dt or u.numel() != c.nvel or not u.is_contiguous():
+            raise ValueError("input of %d %s values expected" % (c.nvel, dt))
+    if ys is None:
+        ys = [torch.empty_like(u) for u in us]
+    for c, y in zip(ctxs, ys):
+        if y.dtype != dt or y.numel() != c.nvel or not y.is_contiguous():
+            raise ValueError("output of %d %s values expected" % (c.nvel, dt))
+    for c in ctxs:
+        c._stream(us[0])
+    hs = (ctypes.c_void_p * n)(*[c.handle for c in ctxs])
+    up = (ctypes.c_void_p * n)(*[u.data_ptr() for u in us])
+    yp = (ctypes.c_void_p * n)(*[y.data_ptr() for y in ys])
+    L = _lib.lib()
+    check(L.ptq2_a11_apply_batch(ctypes.cast(hs, ctypes.c_void_p), n, ctypes.cast(up, ctypes.c_void_p),
+                                 ctypes.cast(yp, ctypes.c_void_p), flags | DEVICE),
+          "ptq2_a11_apply_batch", ctxs[0].handle)
+    return ys
+
+
 # -------------------------------------------------------------- -a11_op registry
 _A11_OPS = {}
 

@@ -19,7 +19,7 @@ class MGOpts(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int), ("galerkin_coarse", ctypes.c_int),
                 ("coarse_eta_mean", ctypes.c_int), ("fp32_basis", ctypes.c_int), ("rigid_deflation", ctypes.c_int),
                 ("rigid_min_contrast", ctypes.c_double), ("rigid_max_inclusions", ctypes.c_int),
-                ("fused_smoother", ctypes.c_int)]
+                ("fused_smoother", ctypes.c_int), ("autotune", ctypes.c_int)]
 
 ETA_MEANS = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "max": 3, "q1proj": 4}
 
@@ -30,6 +30,10 @@ _SOLVER_SIGS = {
     "ptq2_mg_destroy": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "ptq2_mg_levels": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.c_void_p]),
     "ptq2_mg_eigs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "ptq2_mg_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_void_p]),
+    "ptq2_mg_level_times": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.c_void_p]),
     "ptq2_mg_rigid_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                           ctypes.c_void_p]),
     "ptq2_mg_set_stream": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -72,7 +76,8 @@ class ViscousSolver:
 
     def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000,
                  galerkin_coarse=True, eta_mean="arithmetic", emin_frac=None, emax_frac=None, rigid_deflation=False,
-                 rigid_min_contrast=None, rigid_max_inclusions=None, fp32_basis=False, fused_smoother=None):
+                 rigid_min_contrast=None, rigid_max_inclusions=None, fp32_basis=False, fused_smoother=None,
+                 autotune=None):
         L = _sl()
         o = default_opts()
         o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
@@ -88,6 +93,8 @@ class ViscousSolver:
         o.fp32_basis = 1 if fp32_basis else 0
         if fused_smoother is not None:
             o.fused_smoother = 1 if fused_smoother else 0
+        if autotune is not None:
+            o.autotune = {False: 0, True: 1}.get(autotune, autotune)  # 2 / 3: split / MFMA kernel everywhere
         o.rigid_deflation = 1 if rigid_deflation else 0
         if rigid_min_contrast is not None:
             o.rigid_min_contrast = rigid_min_contrast
@@ -111,6 +118,31 @@ class ViscousSolver:
         hi = np.zeros(self.nlevels - 1)
         check(_sl().ptq2_mg_eigs(self._h, lo.ctypes.data, hi.ctypes.data), "ptq2_mg_eigs")
         return lo, hi
+
+    KERNELS = ("brick", "split", "mfma")  # PTQ2_MG_KERNEL_*
+
+    def tuning(self):
+        """Per smoothing level: (kernel index into KERNELS, fast tile addressing, ms per apply
+        chosen, ms per apply of the default shape) as chosen by the set-up's autotuner."""
+        k = self.nlevels - 1
+        w = np.zeros(k, dtype=np.int32)
+        f = np.zeros(k, dtype=np.int32)
+        ms = np.zeros(k)
+        md = np.zeros(k)
+        check(_sl().ptq2_mg_tuning(self._h, w.ctypes.data, f.ctypes.data, ms.ctypes.data, md.ctypes.data),
+              "ptq2_mg_tuning")
+        return [(int(a), int(b), float(c), float(d)) for a, b, c, d in zip(w, f, ms, md)]
+
+    def level_times(self, r, reps=10):
+        """ms of each level's own work in a V-cycle of r (eager launches, mean of reps)."""
+        import torch
+
+        z = torch.empty_like(r)
+        ms = np.zeros(self.nlevels)
+        self._stream(r)
+        _mgcheck(_sl().ptq2_mg_level_times(self._h, r.data_ptr(), z.data_ptr(), reps, ms.ctypes.data),
+                 "ptq2_mg_level_times", self._h)
+        return ms
 
     def rigid_info(self, labels=False):
         """(inclusions, kept rigid-motion columns[, per-node inclusion labels]) of the

@@ -72,6 +72,37 @@ def test_a11_fp64_sinker_matmult(sinker):
 
 
 @pytest.mark.parametrize("dims", [(20, 20, 20), (9, 4, 5), (1, 1, 3)])
+def test_a11_fp32_mfma_form(dims):
+    """The FP32 operator with the reference gradient as a dense GEMM on the matrix cores
+    (PTQ2_MFMA) against the FP64 oracle (1e-5), MatMult and raw ADD, partial bricks included,
+    next to the sum-factorized FP32 kernel; two applies bitwise equal."""
+    mx, my, mz = dims
+    say("case: A11 FP32 MFMA form %s" % (dims,))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 5)
+    cen, r = M.sphere_inclusions(6, 0.1, 0.2, 9)
+    eta = M.viscosity_inclusions(M.qp_coords(mx, my, mz, c), cen, r, 1e3, 1.0, 10)
+    mask = M.bc_mask(mx, my, mz, "freeslip_freesurface")
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    ctx.set_viscosity(eta)
+    n = 3 * M.nnodes(mx, my, mz)
+    u = M.uniform(17, n, -1.0, 1.0)
+    y0 = M.uniform(18, n, -1.0, 1.0)
+    ref = O.matmult_a11(mx, my, mz, c, eta, mask, u)
+    raw = y0 + O.a11_tensor(mx, my, mz, c, eta, u, threads=8)
+    u32, y32 = u.astype(np.float32), y0.astype(np.float32)
+    for fl in (0, _lib.MFMA):
+        ym = ctx.a11(u32, flags=_lib.MATMULT | _lib.FP32 | fl)
+        ya = ctx.a11(u32, y32.copy(), flags=_lib.ADD | _lib.FP32 | fl)
+        say("  flags %#x: MatMult rel %.2e, raw ADD rel %.2e" % (fl, rel(ym, ref), rel(ya, raw)))
+        assert rel(ym, ref) <= 1e-5 and rel(ya, raw) <= 1e-5
+    assert np.array_equal(ctx.a11(u32, flags=_lib.FP32 | _lib.MFMA), ctx.a11(u32, flags=_lib.FP32 | _lib.MFMA))
+    with pytest.raises(ptq2.PTQ2Error):
+        ctx.a11(u, flags=_lib.MFMA)  # FP64 has no matrix-core form
+    ctx.destroy()
+
+
+@pytest.mark.parametrize("dims", [(20, 20, 20), (9, 4, 5), (1, 1, 3)])
 def test_a11_fp64_geometry_variants(dims):
     """The on-the-fly kernel (default, PTQ2_ONTHEFLY) and the resident-geometry kernel
     (PTQ2_STOREDGEO) against the oracle, MatMult and raw ADD, including partial bricks."""

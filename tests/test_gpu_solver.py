@@ -26,12 +26,15 @@ def make(m, contrast, bc="freeslip", n_incl=3):
     return P, ctx
 
 
-@pytest.mark.parametrize("galerkin", [False, True])
-def test_vcycle_fp64_matches_restatement(galerkin):
+@pytest.mark.parametrize("galerkin,autotune", [(False, 0), (True, 0), (True, 2)])
+def test_vcycle_fp64_matches_restatement(galerkin, autotune):
+    """autotune 2 runs the row-split (three waves per brick) smoother kernel on every level."""
     m = 8
-    say("case: FP64 V-cycle 8^3 (3 levels, galerkin coarse %s) vs CPU restatement" % galerkin)
+    say("case: FP64 V-cycle 8^3 (3 levels, galerkin coarse %s, autotune %d) vs CPU restatement" % (galerkin, autotune))
     P, ctx = make(m, 1e2)
-    S = ViscousSolver(ctx, nlevels=3, smooth_its=4, fp32=False, use_graph=True, galerkin_coarse=galerkin)
+    S = ViscousSolver(ctx, nlevels=3, smooth_its=4, fp32=False, use_graph=True, galerkin_coarse=galerkin,
+                      autotune=autotune)
+    assert [k for k, _, _, _ in S.tuning()] == ([1, 1] if autotune == 2 else [0, 0])
     assert [tuple(r) for r in S.levels()] == [(8, 8, 8), (4, 4, 4), (2, 2, 2)]
     emin, emax = S.eigs()
     lv = G.hierarchy(m, m, m, P.coords, P.mask, P.eta, 3)
@@ -77,13 +80,14 @@ def test_vcycle_coarse_viscosity_rules(mean):
     S.destroy()
 
 
-def test_graph_replays_match_eager():
+@pytest.mark.parametrize("autotune", [0, 2, 3])
+def test_graph_replays_match_eager(autotune):
     m = 8
-    say("case: V-cycle graph replays vs eager launches")
+    say("case: V-cycle graph replays vs eager launches, autotune %d" % autotune)
     P, ctx = make(m, 1e3)
-    Sg = ViscousSolver(ctx, nlevels=3, fp32=True, use_graph=True)
+    Sg = ViscousSolver(ctx, nlevels=3, fp32=True, use_graph=True, autotune=autotune)
     P2, ctx2 = make(m, 1e3)
-    Se = ViscousSolver(ctx2, nlevels=3, fp32=True, use_graph=False)
+    Se = ViscousSolver(ctx2, nlevels=3, fp32=True, use_graph=False, autotune=autotune)
     for k in range(4):
         r = torch.from_numpy(M.uniform(10 + k, P.nvel, -1, 1)).to(DEV)
         zg = Sg.vcycle(r).clone()
@@ -199,7 +203,7 @@ def test_rigid_deflation_vcycle_matches_restatement(fp32):
     D = G.RigidDeflation(lv[0])
     assert D.nc >= 2
     S = ViscousSolver(ctx, nlevels=3, smooth_its=3, fp32=fp32, use_graph=True, galerkin_coarse=False,
-                      rigid_deflation=True)
+                      rigid_deflation=True, autotune=0)
     ni, nk, lab = S.rigid_info(labels=True)
     assert ni == D.nc and nk == len(D.keep) and np.array_equal(lab, D.node_label)
     emin, emax = S.eigs()
@@ -212,7 +216,7 @@ def test_rigid_deflation_vcycle_matches_restatement(fp32):
     say("  inclusions %d, columns kept %d, rel err %.3e" % (ni, nk, err))
     assert err <= (1e-4 if fp32 else 1e-10)
     S2 = ViscousSolver(ctx, nlevels=3, smooth_its=3, fp32=fp32, use_graph=False, galerkin_coarse=False,
-                       rigid_deflation=True)
+                       rigid_deflation=True, autotune=0)
     z2 = S2.vcycle(torch.from_numpy(r).to(DEV)).cpu().numpy()
     assert np.array_equal(z, z2)
     S2.destroy()
@@ -297,3 +301,82 @@ def test_vcycle_non_cubic_partial_bricks(fp32):
     assert err <= (1e-4 if fp32 else 1e-10)
     S.destroy()
     ctx.destroy()
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+@pytest.mark.parametrize("dims", [(16, 16, 16), (12, 8, 20)])
+def test_fused_smoother_bitwise_equal(dims, fp32):
+    """The Chebyshev step with the Jacobi scaling and three-term update fused into the apply's
+    store (fused_smoother) gives the unfused smoother's V-cycle bit for bit: on a mesh of full
+    tiles (16^3) and on one whose coarse level has partial bricks (12x8x20 -> 6x4x10), free-slip
+    walls (Dirichlet dofs on every face), FP64 and FP32 smoothing, graph replay; and a solve
+    takes the same iterations."""
+    mx, my, mz = dims
+    say("case: fused smoother %dx%dx%d fp32=%s" % (mx, my, mz, fp32))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 8)
+    mask = M.bc_mask(mx, my, mz, "freeslip")
+    eta = 10.0 ** M.uniform(9, 27 * mx * my * mz, -1.0, 2.0)
+    r = M.uniform(10, c.size, -1, 1)
+    r[mask > 0] = 0.0
+    out = {}
+    for fused in (False, True):
+        ctx = ptq2.Context(0)
+        ctx.setup_mesh(mx, my, mz, c, mask)
+        ctx.set_viscosity(eta)
+        S = ViscousSolver(ctx, nlevels=3, smooth_its=4, fp32=fp32, use_graph=True, galerkin_coarse=True,
+                          fused_smoother=fused, autotune=0)
+        rd = torch.from_numpy(r).to(DEV)
+        z1 = S.vcycle(rd).clone()
+        z2 = S.vcycle(rd).clone()
+        x, info = S.solve(rd, rtol=1e-8, max_it=300, restart=60)
+        torch.cuda.synchronize()
+        out[fused] = (z1, z2, x.clone(), info["iterations"])
+        S.destroy()
+        ctx.destroy()
+    assert torch.equal(out[True][0], out[False][0]) and torch.equal(out[True][1], out[False][1])
+    assert torch.equal(out[True][0], out[True][1])
+    assert out[True][3] == out[False][3] and torch.equal(out[True][2], out[False][2])
+    say("  bitwise equal, %d iterations" % out[True][3])
+
+
+@pytest.mark.parametrize("fp32", [False, True])
+def test_autotuner_choices_and_level_times(fp32):
+    """The set-up's autotuner: every smoothing level reports a valid launch shape with the
+    measured time of the chosen and the default shape (chosen <= default); the row-split
+    kernel's V-cycle agrees with the one-wave kernels' to rounding (FP64 1e-12, FP32 1e-5) on
+    a mesh with partial bricks; the per-level V-cycle times are positive."""
+    mx, my, mz = 24, 20, 28
+    say("case: autotuner %dx%dx%d fp32=%s" % (mx, my, mz, fp32))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / 56, 12)
+    mask = M.bc_mask(mx, my, mz, "freeslip")
+    eta = 10.0 ** M.uniform(13, 27 * mx * my * mz, -1.0, 2.0)
+    r = M.uniform(14, c.size, -1, 1)
+    r[mask > 0] = 0.0
+    rd = torch.from_numpy(r).to(DEV)
+    zs = {}
+    for at in ((0, 2, 3, 1) if fp32 else (0, 2, 1)):
+        ctx = ptq2.Context(0)
+        ctx.setup_mesh(mx, my, mz, c, mask)
+        ctx.set_viscosity(eta)
+        S = ViscousSolver(ctx, nlevels=3, smooth_its=4, fp32=fp32, galerkin_coarse=True, autotune=at)
+        tune = S.tuning()
+        say("  autotune %d: %s" % (at, tune))
+        assert len(tune) == 2
+        for kind, f, ms, md in tune:
+            assert kind in ((0, 1, 2) if fp32 else (0, 1)) and f in (0, 1)
+            if at in (2, 3):
+                assert kind == at - 1
+            if at == 1:
+                assert 0 < ms <= md * 1.0000001
+        zs[at] = S.vcycle(rd).clone()
+        if at == 1:
+            t = S.level_times(rd, reps=3)
+            say("  level times %s ms" % np.round(t, 4))
+            assert np.all(t > 0) and t.sum() < 1000.0
+        S.destroy()
+        ctx.destroy()
+    tol = 1e-5 if fp32 else 1e-12
+    for at in zs:
+        err = (torch.linalg.norm(zs[at] - zs[0]) / torch.linalg.norm(zs[0])).item()
+        say("  autotune %d vs one-wave kernels: rel %.2e" % (at, err))
+        assert err <= tol

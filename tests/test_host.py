@@ -52,6 +52,7 @@ def test_strerror_and_null_arguments():
     assert L.ptq2_a11_apply(None, None, None, 0) == _lib.ERR_INVALID_ARG
     assert L.ptq2_set_viscosity(None, None, 0, 0) == _lib.ERR_INVALID_ARG
     assert L.ptq2_setup_mesh(None, 1, 1, 1, None, None, None) == _lib.ERR_INVALID_ARG
+    assert L.ptq2_a11_apply_batch(None, 1, None, None, 1) == _lib.ERR_INVALID_ARG
 
 
 def test_solver_default_options_round_trip():
@@ -64,6 +65,7 @@ def test_solver_default_options_round_trip():
     assert (o.emin_frac, o.emax_frac) == (0.2, 1.1)
     assert (o.coarse_max_dofs, o.galerkin_coarse, o.coarse_eta_mean) == (20000, 1, Sv.ETA_MEANS["arithmetic"])
     assert (o.fp32_basis, o.rigid_deflation, o.rigid_min_contrast, o.rigid_max_inclusions) == (0, 0, 100.0, 128)
+    assert (o.fused_smoother, o.autotune) == (0, 1)
     assert sorted(Sv.ETA_MEANS.values()) == [0, 1, 2, 3, 4]
     txt = open(os.path.join(REPO, "include", "ptq2_solver.h")).read()
     for name, v in Sv.ETA_MEANS.items():
