@@ -6,12 +6,14 @@ from the coordinates) and one of k_gather_faces.  Inputs are resident in HBM bef
 timed region.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--size 128] [--no-cpu] [--no-fp32]
-                  [--solve M (default 128, 0 = skip)] [--solve-levels L] [--batch]
+                  [--solve M (default 128, 0 = skip)] [--solve-levels L] [--no-batch]
 
 Besides the apply line, `extra` times the resident-geometry FP64 variant and the FP32 smoother
-operator, and `extra.solve` the viscous solve of the buoyancy problem (20 dense spheres,
+operator, `extra.solve` the viscous solve of the buoyancy problem (20 dense spheres,
 viscosity contrast 1e4, free-slip walls) with FP64- and FP32-smoothed GMG to the same true
-residual (rtol 1e-8), the second half of the benchmark metric.
+residual (rtol 1e-8), the second half of the benchmark metric, `extra.batch_apply` the A11 of the
+8 uneven batched problems in one launch against 8 launches, and `extra.batch` their solves
+(24^3 ... 80^3, contrasts 1e2 ... 1e6), one after another and concurrently.
 """
 import argparse
 import json
@@ -201,7 +203,7 @@ def cpu_baseline(seconds=8.0):
 
 
 def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=0.05, rigid=False,
-                     variants=("fp64", "fp32", "mixed"), fused_smoother=None, autotune=None):
+                     variants=("fp64", "fp32", "mixed"), fused_smoother=None, autotune=None, galerkin_levels=0):
     """The viscous solve of the buoyancy problem (free-slip walls, 20 dense spheres with
     viscosity contrast 1e4) with GMG-preconditioned FGMRES: FP64 throughout ("fp64"), FP32
     Chebyshev smoothers ("fp32"), and FP32 smoothers with the finished Arnoldi vectors stored in
@@ -233,7 +235,8 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
             continue
         t = time.perf_counter()
         S = ViscousSolver(ctx, nlevels=nlevels, smooth_its=4, fp32=fp32, emin_frac=emin_frac, rigid_deflation=rigid,
-                          fp32_basis=b32, fused_smoother=fused_smoother, autotune=autotune)
+                          fp32_basis=b32, fused_smoother=fused_smoother, autotune=autotune,
+                          galerkin_levels=galerkin_levels)
         torch.cuda.synchronize()
         setup = time.perf_counter() - t
         S.vcycle(bd)  # warm the graph
@@ -270,7 +273,7 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
 
 
 def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1500, restart=256, rigid=True,
-                     emin_frac=0.05):
+                     emin_frac=0.05, members=None, galerkin_levels=0):
     """Independent buoyancy solves of uneven sizes (viscosity contrasts 1e2..1e6, seeded
     sphere placement, FP32 smoothing), run one after another and then concurrently (one
     stream and host thread per problem).  Host wall time of each variant.  Per problem also
@@ -284,16 +287,20 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
     from ptq2.solver import ViscousSolver, auto_levels, solve_batch
 
     contrasts = [10.0 ** (2 + 4 * k / max(1, len(sizes) - 1)) for k in range(len(sizes))]
+    keep = list(range(len(sizes))) if members is None else list(members)
     solvers, rhs, ctxs = [], [], []
     t = time.perf_counter()
     for k, (m, c) in enumerate(zip(sizes, contrasts)):
+        if k not in keep:
+            continue
         P, b = M.config_buoyancy(m, contrast=c, seed_spheres=100 + k)
         ctx = ptq2.Context(0)
         ctx.setup_mesh(m, m, m, P.coords, P.mask)
         ctx.set_viscosity(P.eta)
         ctxs.append(ctx)
-        solvers.append(ViscousSolver(ctx, nlevels=auto_levels(m), smooth_its=4, fp32=True, rigid_deflation=rigid,
-                                     emin_frac=emin_frac))
+        nl = auto_levels(m)
+        solvers.append(ViscousSolver(ctx, nlevels=nl, smooth_its=4, fp32=True, rigid_deflation=rigid,
+                                     emin_frac=emin_frac, galerkin_levels=min(galerkin_levels, nl - 2)))
         rhs.append(torch.from_numpy(b).to("cuda:0"))
     print("batch set-up %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     for S, b in zip(solvers, rhs):
@@ -314,9 +321,11 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
         floor.append(float("%.3g" % (torch.linalg.norm(y1 - y2).item() / torch.linalg.norm(b).item())))
         del y1, y2
     out = {"workload": "8 buoyancy solves (20 spheres, rho 1.1, free-slip walls), FGMRES(%d) + GMG V-cycle "
-                       "(Chebyshev(4)-Jacobi FP32 on [%g, 1.1] lambda_max)%s, rtol %g, at most %d iterations"
-                       % (restart, emin_frac, ", rigid-body deflation of the inclusions" if rigid else "", rtol, max_it),
-           "sizes": list(sizes), "contrasts": [float("%.3g" % c) for c in contrasts],
+                       "(Chebyshev(4)-Jacobi FP32 on [%g, 1.1] lambda_max)%s%s, rtol %g, at most %d iterations"
+                       % (restart, emin_frac, ", rigid-body deflation of the inclusions" if rigid else "",
+                          ", %d Galerkin levels above the coarsest" % galerkin_levels if galerkin_levels else "",
+                          rtol, max_it),
+           "sizes": [sizes[k] for k in keep], "contrasts": [float("%.3g" % contrasts[k]) for k in keep],
            "iterations": [i["iterations"] for i in seq], "rel_residual": [i["rel"] for i in seq],
            "fp64_residual_floor": floor,
            "batched_iterations_equal": [a["iterations"] == b[1]["iterations"] for a, b in zip(seq, bat)],
@@ -391,9 +400,15 @@ def main():
                     help="Chebyshev steps with the update fused into the apply's store (default: the library's)")
     ap.add_argument("--autotune", type=int, default=None, metavar="0|1|2",
                     help="solver launch-shape autotuner (default: the library's, 1); 2 / 3 = row-split / MFMA kernel everywhere")
+    ap.add_argument("--galerkin-levels", type=int, default=0, metavar="K",
+                    help="Galerkin (assembled stencil) operators on the K levels above the coarsest")
+    ap.add_argument("--batch-only", action="store_true", help="run only the batched solves (--batch)")
+    ap.add_argument("--batch-members", default=None, help="comma-separated subset of the 8 batched problems (0..7)")
     ap.add_argument("--solve-only", action="store_true",
                     help="skip the apply timings and the CPU baseline: run the solve comparison only")
-    ap.add_argument("--batch", action="store_true", help="also time 8 uneven solves, sequential vs concurrent")
+    ap.add_argument("--batch", action="store_true", help=argparse.SUPPRESS)  # the default; kept for old commands
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 8 uneven batched solves (C5), sequential vs concurrent")
     ap.add_argument("--rigid", choices=("off", "on", "both"), default="both",
                     help="rigid-body deflation of the stiff inclusions in the 128^3 solve comparison (both: "
                          "without it as 'solve', with it as 'solve_rigid')")
@@ -413,6 +428,7 @@ def main():
                     help="time the FP64 apply only (no variants, comparisons or CPU baseline): the "
                          "command of the profiler runs")
     args = ap.parse_args()
+    args.batch = not args.no_batch
     if args.apply_only:
         args.no_cpu = args.no_fp32 = True
         args.solve = args.block = args.batch_apply = 0
@@ -442,6 +458,16 @@ def main():
         print(json.dumps({"batch_apply_only": True, "extra": extra}), flush=True)
         return
 
+    if args.batch_only:
+        members = None if args.batch_members is None else [int(v) for v in args.batch_members.split(",")]
+        extra = {}
+        for rigid in {"off": [False], "on": [True], "both": [False, True]}[args.batch_rigid]:
+            key = "batch_plain" if not rigid and args.batch_rigid == "both" else "batch"
+            extra[key] = batch_comparison(rigid=rigid, members=members, galerkin_levels=args.galerkin_levels,
+                                          max_it=args.solve_max_it if args.solve_max_it != 1000 else 1500)
+        print(json.dumps({"batch_only": True, "extra": extra}), flush=True)
+        return
+
     if args.solve_only:
         rigid = {"off": [False], "on": [True], "both": [False, True]}[args.rigid]
         extra = {}
@@ -449,7 +475,8 @@ def main():
             key = "solve_rigid" if rg and args.rigid == "both" else "solve"
             extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rg, max_it=args.solve_max_it,
                                           variants=tuple(args.solve_variants.split(",")),
-                                          fused_smoother=args.fused_smoother, autotune=args.autotune)
+                                          fused_smoother=args.fused_smoother, autotune=args.autotune,
+                                          galerkin_levels=args.galerkin_levels)
         print(json.dumps({"solve_only": True, "extra": extra}), flush=True)
         return
 
@@ -514,7 +541,8 @@ def main():
         try:
             extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rigid, max_it=args.solve_max_it,
                                           variants=tuple(args.solve_variants.split(",")),
-                                          fused_smoother=args.fused_smoother, autotune=args.autotune)
+                                          fused_smoother=args.fused_smoother, autotune=args.autotune,
+                                          galerkin_levels=args.galerkin_levels)
         except Exception as exc:  # the apply line above stays valid; report the failure
             extra[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     for rigid in variants[args.batch_rigid] if args.batch else []:

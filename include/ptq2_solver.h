@@ -70,6 +70,13 @@ typedef struct {
      every level, 3: the matrix-core form on every level (FP32 smoothing), untimed (for
      tests and comparisons). */
   int autotune;
+  /* k >= 0 (default 0): the k levels above the coarsest carry the Galerkin operator
+     Pi R A P Pi + (I - Pi) of the level above them instead of the rediscretised matrix-free
+     operator (the reference's Galerkin coarse levels, src/ptatin_driver_linear_ts.c:64-385),
+     assembled on the device by coloured probing as 5x5x5 stencils of 3x3 blocks
+     (1125 values per node, FP64 plus an FP32 copy with FP32 smoothing).  0 <= k <=
+     nlevels - 2; level 0 stays matrix-free.  For high viscosity contrasts. */
+  int galerkin_levels;
 } ptq2_mg_opts;
 
 /* PTQ2_ETA_MEAN_Q1PROJ: the reference's coefficient hierarchy
@@ -117,7 +124,7 @@ int ptq2_mg_rigid_info(const ptq2_mg *mg, int *ninclusions, int *ncolumns, int32
 /* the Chebyshev interval [emin, emax] of every smoothed level (nlevels-1 entries each) */
 int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax);
 /* Kernels of the smoother apply (ptq2_mg_tuning) */
-enum { PTQ2_MG_KERNEL_BRICK = 0, PTQ2_MG_KERNEL_SPLIT = 1, PTQ2_MG_KERNEL_MFMA = 2 };
+enum { PTQ2_MG_KERNEL_BRICK = 0, PTQ2_MG_KERNEL_SPLIT = 1, PTQ2_MG_KERNEL_MFMA = 2, PTQ2_MG_KERNEL_STENCIL = 3 };
 /* Launch shape the autotuner chose for the smoother apply of each level l < nlevels - 1:
    kernel (PTQ2_MG_KERNEL_*), tile addressing (1: fast full-tile path, 0: general), and the
    measured ms per apply of the chosen and of the default shape (both 0 when autotune was off

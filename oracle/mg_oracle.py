@@ -266,7 +266,51 @@ class Level:
         return O.matgetdiag_a11(self.mx, self.my, self.mz, self.coords, self.eta, self.mask)
 
 
-def hierarchy(mx, my, mz, coords, mask, eta, nlevels, mode="arithmetic"):
+class GalerkinLevel:
+    """Level C with the Galerkin operator Pi R A_F P Pi + (I - Pi) of the level F above it
+    (Pi zeroes C's Dirichlet dofs; A_F with MatMult semantics), assembled column by column
+    into a sparse matrix: the restatement of ptq2_mg_opts.galerkin_levels (the reference's
+    Galerkin coarse levels, src/ptatin_driver_linear_ts.c:64-385, with PETSc's MatPtAP)."""
+
+    def __init__(self, F, C, P=None, R=None):
+        import scipy.sparse as sp
+
+        P = P or prolong
+        R = R or restrict
+        self.mx, self.my, self.mz = C.mx, C.my, C.mz
+        self.coords, self.mask, self.eta = C.coords, C.mask, C.eta
+        self.n = n = C.n
+        dmask = np.zeros(n, dtype=bool) if C.mask is None else C.mask > 0
+        rows, cols, vals = [], [], []
+        for j in range(n):
+            if dmask[j]:
+                rows.append(np.array([j]))
+                cols.append(np.array([j]))
+                vals.append(np.array([1.0]))
+                continue
+            e = np.zeros(n)
+            e[j] = 1.0
+            v = P(C.mx, C.my, C.mz, e)
+            if F.mask is not None:
+                v[F.mask > 0] = 0.0
+            col = R(C.mx, C.my, C.mz, F.matmult(v))
+            col[dmask] = 0.0
+            nz = np.nonzero(col)[0]
+            rows.append(nz)
+            cols.append(np.full(nz.size, j))
+            vals.append(col[nz])
+        self.A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+    def matmult(self, x):
+        return self.A @ x
+
+    def diag(self):
+        return self.A.diagonal().copy()
+
+
+def hierarchy(mx, my, mz, coords, mask, eta, nlevels, mode="arithmetic", galerkin_levels=0):
+    """Levels 0 .. nlevels-1; with galerkin_levels = k, levels nlevels-1-k .. nlevels-2 carry the
+    Galerkin operator of the level above them (GalerkinLevel), built from the top down."""
     lv = [Level(mx, my, mz, coords, mask, eta)]
     proj = q1_projection_hierarchy(mx, my, mz, eta, nlevels) if mode == "q1proj" else None
     for l in range(nlevels - 1):
@@ -274,6 +318,8 @@ def hierarchy(mx, my, mz, coords, mask, eta, nlevels, mode="arithmetic"):
         mask = None if f.mask is None else inject_from_fine(f, f.mask)
         ec = proj[l] if proj is not None else eta_coarsen(f.mx, f.my, f.mz, f.eta, mode)
         c = Level(f.mx // 2, f.my // 2, f.mz // 2, inject_from_fine(f), mask, ec)
+        if l + 1 >= nlevels - 1 - galerkin_levels and l + 1 < nlevels - 1:
+            c = GalerkinLevel(f, c)
         lv.append(c)
     return lv
 

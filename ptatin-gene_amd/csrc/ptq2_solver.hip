@@ -952,6 +952,135 @@ __global__ void k_rg_gather(int64_t nd, const int *__restrict__ dnode, const int
   }
 }
 
+// column j of the operator assembled into row j of A (ld columns): e = unit vector j
+__global__ void k_unit(int64_t n, int64_t j, double *__restrict__ e) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) e[i] = i == j ? 1.0 : 0.0;
+}
+__global__ void k_pad_identity(int nc, int np, double *__restrict__ A) {
+  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (int64_t)np * np) return;
+  int i = (int)(idx / np), j = (int)(idx % np);
+  if (i >= nc || j >= nc) A[idx] = i == j ? 1.0 : 0.0;
+}
+// rows of coarse Dirichlet dofs: the unit row (the MatMult row y_D = x_D)
+__global__ void k_dirichlet_unit_rows(int64_t n, const uint8_t *__restrict__ mask, double *__restrict__ A, int ld) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && mask && mask[i]) A[i * ld + i] = 1.0;
+}
+// the dof of the probed colour class (node indices = (cx,cy,cz) modulo pp on each axis,
+// component comp) nearest to node (I,J,K): offsets -pr..pr per axis, -1 if outside the box
+__device__ __forceinline__ int64_t probe_member(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
+                                                int I, int J, int K) {
+  const int pr = pp >> 1;
+  int d = (cx - I % pp + pp) % pp, Ij = I + (d <= pr ? d : d - pp);
+  d = (cy - J % pp + pp) % pp;
+  int Jj = J + (d <= pr ? d : d - pp);
+  d = (cz - K % pp + pp) % pp;
+  int Kj = K + (d <= pr ? d : d - pp);
+  if (Ij < 0 || Ij >= NX || Jj < 0 || Jj >= NY || Kj < 0 || Kj >= NZ) return -1;
+  return 3 * ((int64_t)Ij + (int64_t)NX * ((int64_t)Jj + (int64_t)NY * Kj)) + comp;
+}
+// probe vector of a colour class: 1 on its non-Dirichlet dofs, 0 elsewhere
+__global__ void k_probe_vec(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
+                            const uint8_t *__restrict__ mask, double *__restrict__ e) {
+  const int64_t n = 3 * (int64_t)NX * NY * NZ;
+  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < n; d += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = d / 3;
+    const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / ((int64_t)NX * NY));
+    const bool on = (int)(d - 3 * node) == comp && I % pp == cx && J % pp == cy && K % pp == cz;
+    e[d] = on && !(mask && mask[d]) ? 1.0 : 0.0;
+  }
+}
+// entry r of the probe result belongs to the column of the class member next to dof r's node:
+// row j of A holds column j of the operator (A[j][r] = (A e_j)[r])
+__global__ void k_probe_scatter(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
+                                const uint8_t *__restrict__ mask, const double *__restrict__ w, double *__restrict__ A,
+                                int ld) {
+  const int64_t n = 3 * (int64_t)NX * NY * NZ;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t node = r / 3;
+    const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / ((int64_t)NX * NY));
+    const int64_t j = probe_member(NX, NY, NZ, pp, cx, cy, cz, comp, I, J, K);
+    if (j >= 0 && !(mask && mask[j])) A[j * ld + r] = w[r];
+  }
+}
+// Galerkin levels as assembled stencils.  The Galerkin product of the trilinear transfers
+// around a Q2 operator couples a node to the nodes within two nodes per axis, so a level's
+// operator is a 5x5x5 stencil of 3x3 blocks per node: A[((o * 3 + a) * 3 + b) * nn + p] couples
+// component a of node p to component b of node p + offset(o), o = (dz+2) 25 + (dy+2) 5 + (dx+2)
+// (node-fastest, so a thread per node reads it coalesced).  Entries towards nodes outside the
+// box are zero.
+constexpr int SN = 125, SC = 62;  // stencil offsets, the centre
+// entry (p, a) of a probe result (colour class (cx,cy,cz) of period 5, component comp) is the
+// coupling of p to the class member next to it
+__global__ void k_stencil_scatter(int NX, int NY, int NZ, int cx, int cy, int cz, int comp,
+                                  const uint8_t *__restrict__ mask, const double *__restrict__ w,
+                                  double *__restrict__ A) {
+  const int64_t nn = (int64_t)NX * NY * NZ;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nn; p += (int64_t)gridDim.x * blockDim.x) {
+    const int I = (int)(p % NX), J = (int)((p / NX) % NY), K = (int)(p / ((int64_t)NX * NY));
+    const int64_t q = probe_member(NX, NY, NZ, 5, cx, cy, cz, comp, I, J, K);
+    if (q < 0 || (mask && mask[q])) continue;
+    const int64_t qn = q / 3;
+    const int Iq = (int)(qn % NX), Jq = (int)((qn / NX) % NY), Kq = (int)(qn / ((int64_t)NX * NY));
+    const int o = (Kq - K + 2) * 25 + (Jq - J + 2) * 5 + (Iq - I + 2);
+#pragma unroll
+    for (int a = 0; a < 3; a++) A[((int64_t)(o * 3 + a) * 3 + comp) * nn + p] = w[3 * p + a];
+  }
+}
+// Dirichlet dofs: unit diagonal (their rows and columns are zero after the probing)
+__global__ void k_stencil_dirichlet(int64_t nn, const uint8_t *__restrict__ mask, double *__restrict__ A) {
+  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < 3 * nn; d += (int64_t)gridDim.x * blockDim.x) {
+    if (!mask[d]) continue;
+    const int64_t p = d / 3;
+    const int a = (int)(d - 3 * p);
+    A[((int64_t)(SC * 3 + a) * 3 + a) * nn + p] = 1.0;
+  }
+}
+__global__ void k_stencil_diag(int64_t nn, const double *__restrict__ A, double *__restrict__ d) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 3 * nn; i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t p = i / 3;
+    const int a = (int)(i - 3 * p);
+    d[i] = A[((int64_t)(SC * 3 + a) * 3 + a) * nn + p];
+  }
+}
+// y = A x (MatMult semantics: the Dirichlet rows are unit rows), one thread per node, the 125
+// neighbours in a fixed order
+template <typename T>
+__global__ void k_stencil_apply(int NX, int NY, int NZ, const T *__restrict__ A, const T *__restrict__ x,
+                                T *__restrict__ y) {
+  const int64_t nn = (int64_t)NX * NY * NZ;
+  for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < nn; p += (int64_t)gridDim.x * blockDim.x) {
+    const int I = (int)(p % NX), J = (int)((p / NX) % NY), K = (int)(p / ((int64_t)NX * NY));
+    T a0 = T(0), a1 = T(0), a2 = T(0);
+    for (int dz = -2; dz <= 2; dz++) {
+      if (K + dz < 0 || K + dz >= NZ) continue;
+      for (int dy = -2; dy <= 2; dy++) {
+        if (J + dy < 0 || J + dy >= NY) continue;
+#pragma unroll
+        for (int dx = -2; dx <= 2; dx++) {
+          if (I + dx < 0 || I + dx >= NX) continue;
+          const int64_t q = p + dx + (int64_t)NX * (dy + (int64_t)NY * dz);
+          const int o = (dz + 2) * 25 + (dy + 2) * 5 + (dx + 2);
+          const T *Ao = A + (int64_t)o * 9 * nn + p;
+          const T x0 = x[3 * q], x1 = x[3 * q + 1], x2 = x[3 * q + 2];
+          a0 = fma(Ao[0], x0, fma(Ao[nn], x1, fma(Ao[2 * nn], x2, a0)));
+          a1 = fma(Ao[3 * nn], x0, fma(Ao[4 * nn], x1, fma(Ao[5 * nn], x2, a1)));
+          a2 = fma(Ao[6 * nn], x0, fma(Ao[7 * nn], x1, fma(Ao[8 * nn], x2, a2)));
+        }
+      }
+    }
+    y[3 * p] = a0;
+    y[3 * p + 1] = a1;
+    y[3 * p + 2] = a2;
+  }
+}
+__global__ void k_round_f32(int64_t n, const double *__restrict__ a, float *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = (float)a[i];
+}
+
 struct DevBuf {
   void *p = nullptr;
   ~DevBuf() { if (p) (void)hipFree(p); }
@@ -970,6 +1099,9 @@ struct LevelTune {
 struct ptq2_mg {
   ptq2_mg_opts opt{};
   std::vector<LevelTune> tune;          // per level (smoothing levels)
+  std::vector<char> gal;                // per level: Galerkin stencil operator (galerkin_levels)
+  std::vector<double *> gst64;          // its stencil in FP64 (probing, FP64 applies)
+  std::vector<float *> gst32;           // and in FP32 (FP32 smoothing)
   std::vector<ptq2::Box> lbox;          // per level: the box of the level's launches (tuned fast32)
   bool prof = false;                    // record per-level events in the eager V-cycle
   std::vector<hipEvent_t> pev;          // 4 per level: entry, before / after the coarser level, exit
@@ -1044,6 +1176,11 @@ int level_matmult(ptq2_mg *mg, int l, const T *x, T *y);
 hipError_t level_launch(ptq2_mg *mg, int l, const double *x, double *y, bool gather) {
   ptq2_ctx *c = mg->ctx[l];
   const ptq2::Box &b = mg->lbox[l];
+  if (mg->gal[l]) {
+    hipLaunchKernelGGL(k_stencil_apply<double>, dim3(nblk(c->nnodes)), dim3(TPB), 0, mg->stream, b.NX, b.NY, b.NZ,
+                       (const double *)mg->gst64[l], x, y);
+    return hipGetLastError();
+  }
   // geometry recomputed from the coordinates (the faster FP64 kernel); the level's face
   // buffer was allocated by its set-up applies (no allocation here: this runs under graph
   // capture)
@@ -1056,6 +1193,11 @@ hipError_t level_launch(ptq2_mg *mg, int l, const double *x, double *y, bool gat
 hipError_t level_launch(ptq2_mg *mg, int l, const float *x, float *y, bool gather) {
   ptq2_ctx *c = mg->ctx[l];
   const ptq2::Box &b = mg->lbox[l];
+  if (mg->gal[l]) {
+    hipLaunchKernelGGL(k_stencil_apply<float>, dim3(nblk(c->nnodes)), dim3(TPB), 0, mg->stream, b.NX, b.NY, b.NZ,
+                       (const float *)mg->gst32[l], x, y);
+    return hipGetLastError();
+  }
   if (mg->tune[l].split == 1 && mg->f32)
     return ptq2::launch_a11_fp32_split(b, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, gather);
   if (mg->tune[l].split == 2 && mg->f32)
@@ -1123,7 +1265,8 @@ int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
   // one Chebyshev step from xin: the apply, then the update.  Fused: the apply's store
   // updates the final owned dofs and the face pass the lower-face ones; otherwise the apply
   // stores A xin and the face pass updates every dof.
-  const bool fz = mg->opt.fused_smoother != 0 && lc->facebuf && mg->tune[l].split == 0;
+  const bool fz = mg->opt.fused_smoother != 0 && lc->facebuf && mg->tune[l].split == 0 && !mg->gal[l];
+  const T *face = mg->gal[l] ? nullptr : (const T *)lc->facebuf;  // a stencil apply is complete
   auto step = [&](const T *xin, const T *pkm1, T *out, double cr, double ck, double cx) -> int {
     const ptq2::ChebOut<T> co{(const T *)b, (const T *)dinv, xin, pkm1, out, (const uint8_t *)lc->mask, (T)cr, (T)ck,
                               (T)cx};
@@ -1132,7 +1275,7 @@ int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
     else
       MGRC(level_apply_owned<T>(mg, l, xin, y));
     hipLaunchKernelGGL(k_cheb_faces<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, lc->box, n, (const T *)b,
-                       (const T *)y, (const T *)dinv, xin, pkm1, out, (const T *)lc->facebuf,
+                       (const T *)y, (const T *)dinv, xin, pkm1, out, face,
                        (const uint8_t *)lc->mask, (T)cr, (T)ck, (T)cx, fz ? 1 : 0);
     return PTQ2_OK;
   };
@@ -1190,6 +1333,60 @@ int vcycle_level(ptq2_mg *mg, int l) {
   return hipGetLastError() == hipSuccess ? PTQ2_OK : mgfail(mg, PTQ2_ERR_DEVICE, "v-cycle launch");
 }
 
+// The Galerkin operator Pi R A P Pi + (I - Pi) of level l through level l - 1 (FP64 MatMult of
+// level l-1: its matrix-free operator or its own stencil), assembled by coloured probing: nodes
+// whose indices agree modulo 5 on every axis are more than 4 apart, so they couple to disjoint
+// stencil rows, and one probe per class and component (375) gives every entry.
+int build_galerkin_level(ptq2_mg *mg, int l) {
+  ptq2_ctx *c = mg->ctx[l], *f = mg->ctx[l - 1];
+  const int64_t nn = c->nnodes, nc = c->nvel, nf = f->nvel;
+  const size_t ne = (size_t)nn * SN * 9;
+  MGRC(mg_alloc(mg, (void **)&mg->gst64[l], sizeof(double) * ne));
+  if (mg->f32) MGRC(mg_alloc(mg, (void **)&mg->gst32[l], sizeof(float) * ne));
+  double *e = nullptr, *w = nullptr, *vf = nullptr, *wf = nullptr;
+  int rc = mg_alloc(mg, (void **)&e, sizeof(double) * nc);
+  if (!rc) rc = mg_alloc(mg, (void **)&w, sizeof(double) * nc);
+  if (!rc) rc = mg_alloc(mg, (void **)&vf, sizeof(double) * nf);
+  if (!rc) rc = mg_alloc(mg, (void **)&wf, sizeof(double) * nf);
+  const int *mc = &mg->m[3 * l], *mf = &mg->m[3 * (l - 1)];
+  const int NXc = 2 * mc[0] + 1, NYc = 2 * mc[1] + 1, NZc = 2 * mc[2] + 1;
+  const int NXf = 2 * mf[0] + 1, NYf = 2 * mf[1] + 1, NZf = 2 * mf[2] + 1;
+  if (!rc) hipLaunchKernelGGL(k_fill<double>, dim3(nblk((int64_t)ne)), dim3(TPB), 0, mg->stream, (int64_t)ne,
+                              mg->gst64[l], 0.0);
+  int probes = 0;
+  for (int cz = 0; cz < 5 && !rc; cz++)
+    for (int cy = 0; cy < 5 && !rc; cy++)
+      for (int cx = 0; cx < 5 && !rc; cx++)
+        for (int comp = 0; comp < 3 && !rc; comp++) {
+          hipLaunchKernelGGL(k_probe_vec, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, 5, cx, cy, cz,
+                             comp, (const uint8_t *)c->mask, e);
+          hipLaunchKernelGGL(k_fill<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, nf, vf, 0.0);
+          hipLaunchKernelGGL(k_prolong_add<double>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, NXc, NYc, NXf, NYf,
+                             NZf, (const double *)e, (const uint8_t *)f->mask, vf);
+          rc = level_matmult<double>(mg, l - 1, vf, wf);
+          hipLaunchKernelGGL(k_restrict<double>, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, NXf, NYf,
+                             NZf, (const double *)wf, (const double *)nullptr, (const uint8_t *)c->mask, w);
+          hipLaunchKernelGGL(k_stencil_scatter, dim3(nblk(nn)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, cx, cy, cz,
+                             comp, (const uint8_t *)c->mask, (const double *)w, mg->gst64[l]);
+          if (!rc && hipGetLastError() != hipSuccess) rc = mgfail(mg, PTQ2_ERR_DEVICE, "galerkin probe launch");
+          // bound the work queued ahead of the host
+          if (!rc && ++probes % 32 == 0 && hipStreamSynchronize(mg->stream) != hipSuccess)
+            rc = mgfail(mg, PTQ2_ERR_DEVICE, "galerkin probing");
+        }
+  if (!rc && c->mask)
+    hipLaunchKernelGGL(k_stencil_dirichlet, dim3(nblk(nc)), dim3(TPB), 0, mg->stream, nn,
+                       (const uint8_t *)c->mask, mg->gst64[l]);
+  if (!rc && mg->f32)
+    hipLaunchKernelGGL(k_round_f32, dim3(nblk((int64_t)ne)), dim3(TPB), 0, mg->stream, (int64_t)ne,
+                       (const double *)mg->gst64[l], mg->gst32[l]);
+  if (!rc && (hipGetLastError() != hipSuccess || hipStreamSynchronize(mg->stream) != hipSuccess))
+    rc = mgfail(mg, PTQ2_ERR_DEVICE, "galerkin level");
+  for (double *p : {e, w, vf, wf})
+    if (p) (void)hipFree(p);
+  if (!rc) mg->gal[l] = 1;
+  return rc;
+}
+
 // ms per apply of level l's smoother apply (no face pass) in the current launch shape: the
 // fastest of 3 timed batches of reps launches, after 2 untimed ones
 template <typename T>
@@ -1231,7 +1428,7 @@ int autotune(ptq2_mg *mg) {
   for (int l = 0; l + 1 < mg->nl; l++) {
     ptq2_ctx *c = mg->ctx[l];
     LevelTune &t = mg->tune[l];
-    if (!mg->opt.autotune || !c->facebuf) continue;
+    if (!mg->opt.autotune || !c->facebuf || mg->gal[l]) continue;
     if (mg->opt.autotune >= 2) {  // one kernel everywhere, untimed
       t.split = mg->opt.autotune == 2 ? 1 : (sizeof(T) == 4 ? 2 : 0);
       continue;
@@ -1396,59 +1593,7 @@ int maxpy_norm(ptq2_mg *mg, int nv, double *t, int64_t n, double *nrm) {
 // result is deterministic.
 constexpr int SB = 64;
 
-// column j of the operator assembled into row j of A (ld columns): e = unit vector j
-__global__ void k_unit(int64_t n, int64_t j, double *__restrict__ e) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) e[i] = i == j ? 1.0 : 0.0;
-}
-__global__ void k_pad_identity(int nc, int np, double *__restrict__ A) {
-  int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (int64_t)np * np) return;
-  int i = (int)(idx / np), j = (int)(idx % np);
-  if (i >= nc || j >= nc) A[idx] = i == j ? 1.0 : 0.0;
-}
-// rows of coarse Dirichlet dofs: the unit row (the MatMult row y_D = x_D)
-__global__ void k_dirichlet_unit_rows(int64_t n, const uint8_t *__restrict__ mask, double *__restrict__ A, int ld) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && mask && mask[i]) A[i * ld + i] = 1.0;
-}
-// the dof of the probed colour class (node indices = (cx,cy,cz) modulo pp on each axis,
-// component comp) nearest to node (I,J,K): offsets -pr..pr per axis, -1 if outside the box
-__device__ __forceinline__ int64_t probe_member(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
-                                                int I, int J, int K) {
-  const int pr = pp >> 1;
-  int d = (cx - I % pp + pp) % pp, Ij = I + (d <= pr ? d : d - pp);
-  d = (cy - J % pp + pp) % pp;
-  int Jj = J + (d <= pr ? d : d - pp);
-  d = (cz - K % pp + pp) % pp;
-  int Kj = K + (d <= pr ? d : d - pp);
-  if (Ij < 0 || Ij >= NX || Jj < 0 || Jj >= NY || Kj < 0 || Kj >= NZ) return -1;
-  return 3 * ((int64_t)Ij + (int64_t)NX * ((int64_t)Jj + (int64_t)NY * Kj)) + comp;
-}
-// probe vector of a colour class: 1 on its non-Dirichlet dofs, 0 elsewhere
-__global__ void k_probe_vec(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
-                            const uint8_t *__restrict__ mask, double *__restrict__ e) {
-  const int64_t n = 3 * (int64_t)NX * NY * NZ;
-  for (int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; d < n; d += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t node = d / 3;
-    const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / ((int64_t)NX * NY));
-    const bool on = (int)(d - 3 * node) == comp && I % pp == cx && J % pp == cy && K % pp == cz;
-    e[d] = on && !(mask && mask[d]) ? 1.0 : 0.0;
-  }
-}
-// entry r of the probe result belongs to the column of the class member next to dof r's node:
-// row j of A holds column j of the operator (A[j][r] = (A e_j)[r])
-__global__ void k_probe_scatter(int NX, int NY, int NZ, int pp, int cx, int cy, int cz, int comp,
-                                const uint8_t *__restrict__ mask, const double *__restrict__ w, double *__restrict__ A,
-                                int ld) {
-  const int64_t n = 3 * (int64_t)NX * NY * NZ;
-  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t node = r / 3;
-    const int I = (int)(node % NX), J = (int)((node / NX) % NY), K = (int)(node / ((int64_t)NX * NY));
-    const int64_t j = probe_member(NX, NY, NZ, pp, cx, cy, cz, comp, I, J, K);
-    if (j >= 0 && !(mask && mask[j])) A[j * ld + r] = w[r];
-  }
-}
+
 // symmetrise the round-off of the assembled operator
 __global__ void k_symmetrise(int np, double *__restrict__ A) {
   int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1849,6 +1994,7 @@ void ptq2_mg_default_opts(ptq2_mg_opts *o) {
   o->rigid_max_inclusions = RG_MAXINC;
   o->fused_smoother = 0;
   o->autotune = 1;
+  o->galerkin_levels = 0;
 }
 
 int ptq2_mg_destroy(ptq2_mg **pmg) {
@@ -1857,6 +2003,10 @@ int ptq2_mg_destroy(ptq2_mg **pmg) {
   if (mg->stream) (void)hipStreamSynchronize(mg->stream);
   for (hipEvent_t e : mg->pev)
     if (e) (void)hipEventDestroy(e);
+  for (double *p : mg->gst64)
+    if (p) (void)hipFree(p);
+  for (float *p : mg->gst32)
+    if (p) (void)hipFree(p);
   if (mg->exec) (void)hipGraphExecDestroy(mg->exec);
   if (mg->graph) (void)hipGraphDestroy(mg->graph);
   for (size_t l = 0; l < mg->vb.size(); l++) {
@@ -1890,7 +2040,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   if (opts) o = *opts;
   else ptq2_mg_default_opts(&o);
   if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0) ||
-      o.autotune < 0 || o.autotune > 3 ||
+      o.autotune < 0 || o.autotune > 3 || o.galerkin_levels < 0 || o.galerkin_levels > o.nlevels - 2 ||
       o.coarse_eta_mean < PTQ2_ETA_MEAN_ARITHMETIC || o.coarse_eta_mean > PTQ2_ETA_MEAN_Q1PROJ)
     return PTQ2_ERR_INVALID_ARG;
   if (o.rigid_deflation &&
@@ -1991,6 +2141,16 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     mg->lbox.push_back(mg->ctx[l]->box);
     mg->tune[l].fast = mg->ctx[l]->box.fast32;
   }
+  // ---- Galerkin levels: nl-1-galerkin_levels .. nl-2, each the product through the level above
+  mg->gal.assign(mg->nl, 0);
+  mg->gst64.assign(mg->nl, nullptr);
+  mg->gst32.assign(mg->nl, nullptr);
+  for (int l = 0; l + 1 < mg->nl; l++) {
+    int rc = PTQ2_OK;
+    (void)ptq2_internal_facebuf(mg->ctx[l], &rc);
+    MGRC(rc);
+  }
+  for (int l = mg->nl - 1 - o.galerkin_levels; l + 1 < mg->nl; l++) MGRC(build_galerkin_level(mg, l));
 
   // ---- small reduction buffers (also used by the solve)
   MGRC(mg_alloc(mg, (void **)&mg->part, sizeof(double) * (2 * MAXRESTART + 4) * MDB));
@@ -2019,7 +2179,14 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
     MGRC(mg_alloc(mg, (void **)&d, sizeof(double) * n));
     MGRC(mg_alloc(mg, (void **)&v, sizeof(double) * n));
     MGRC(mg_alloc(mg, (void **)&w, sizeof(double) * n));
-    int rc = ptq2_a11_diag(c, d, PTQ2_DEVICE | PTQ2_MATMULT);
+    int rc = PTQ2_OK;
+    if (mg->gal[l]) {
+      hipLaunchKernelGGL(k_stencil_diag, dim3(nblk(n)), dim3(TPB), 0, mg->stream, c->nnodes,
+                         (const double *)mg->gst64[l], d);
+      if (hipGetLastError() != hipSuccess) rc = mgfail(mg, PTQ2_ERR_DEVICE, "stencil diagonal");
+    } else {
+      rc = ptq2_a11_diag(c, d, PTQ2_DEVICE | PTQ2_MATMULT);
+    }
     if (rc == PTQ2_OK) {
       if (mg->f32) hipLaunchKernelGGL(k_recip<float>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)d, (float *)mg->vdinv[l]);
       else hipLaunchKernelGGL(k_recip<double>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (const double *)d, (double *)mg->vdinv[l]);
@@ -2040,7 +2207,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
       const unsigned nb = 1024;
       std::vector<double> hp(2 * nb);
       for (int it = 0; it < o.esteig_its && rc == PTQ2_OK; it++) {
-        rc = ptq2_a11_apply(c, v, w, PTQ2_DEVICE | PTQ2_MATMULT);
+        rc = mg->gal[l] ? level_matmult<double>(mg, l, v, w) : ptq2_a11_apply(c, v, w, PTQ2_DEVICE | PTQ2_MATMULT);
         if (rc) break;
         hipLaunchKernelGGL(k_rq_parts, dim3(nb), dim3(TPB), 0, mg->stream, n, (const double *)v, (const double *)w,
                            (const double *)d, mg->part);
@@ -2219,7 +2386,7 @@ int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax) {
 int ptq2_mg_tuning(const ptq2_mg *mg, int *kernel, int *fast_tiles, double *ms, double *ms_default) {
   if (!mg) return PTQ2_ERR_INVALID_ARG;
   for (int l = 0; l + 1 < mg->nl; l++) {
-    if (kernel) kernel[l] = mg->tune[l].split;
+    if (kernel) kernel[l] = mg->gal[l] ? PTQ2_MG_KERNEL_STENCIL : mg->tune[l].split;
     if (fast_tiles) fast_tiles[l] = mg->tune[l].fast;
     if (ms) ms[l] = mg->tune[l].ms;
     if (ms_default) ms_default[l] = mg->tune[l].ms_default;

@@ -19,7 +19,7 @@ class MGOpts(ctypes.Structure):
                 ("use_graph", ctypes.c_int), ("coarse_max_dofs", ctypes.c_int), ("galerkin_coarse", ctypes.c_int),
                 ("coarse_eta_mean", ctypes.c_int), ("fp32_basis", ctypes.c_int), ("rigid_deflation", ctypes.c_int),
                 ("rigid_min_contrast", ctypes.c_double), ("rigid_max_inclusions", ctypes.c_int),
-                ("fused_smoother", ctypes.c_int), ("autotune", ctypes.c_int)]
+                ("fused_smoother", ctypes.c_int), ("autotune", ctypes.c_int), ("galerkin_levels", ctypes.c_int)]
 
 ETA_MEANS = {"arithmetic": 0, "geometric": 1, "harmonic": 2, "max": 3, "q1proj": 4}
 
@@ -77,7 +77,7 @@ class ViscousSolver:
     def __init__(self, ctx, nlevels=5, smooth_its=4, fp32=True, esteig_its=20, use_graph=True, coarse_max_dofs=20000,
                  galerkin_coarse=True, eta_mean="arithmetic", emin_frac=None, emax_frac=None, rigid_deflation=False,
                  rigid_min_contrast=None, rigid_max_inclusions=None, fp32_basis=False, fused_smoother=None,
-                 autotune=None):
+                 autotune=None, galerkin_levels=0):
         L = _sl()
         o = default_opts()
         o.nlevels, o.smooth_its, o.fp32_smoother = nlevels, smooth_its, 1 if fp32 else 0
@@ -93,6 +93,7 @@ class ViscousSolver:
         o.fp32_basis = 1 if fp32_basis else 0
         if fused_smoother is not None:
             o.fused_smoother = 1 if fused_smoother else 0
+        o.galerkin_levels = galerkin_levels
         if autotune is not None:
             o.autotune = {False: 0, True: 1}.get(autotune, autotune)  # 2 / 3: split / MFMA kernel everywhere
         o.rigid_deflation = 1 if rigid_deflation else 0
@@ -119,7 +120,7 @@ class ViscousSolver:
         check(_sl().ptq2_mg_eigs(self._h, lo.ctypes.data, hi.ctypes.data), "ptq2_mg_eigs")
         return lo, hi
 
-    KERNELS = ("brick", "split", "mfma")  # PTQ2_MG_KERNEL_*
+    KERNELS = ("brick", "split", "mfma", "stencil")  # PTQ2_MG_KERNEL_*
 
     def tuning(self):
         """Per smoothing level: (kernel index into KERNELS, fast tile addressing, ms per apply

@@ -380,3 +380,46 @@ def test_autotuner_choices_and_level_times(fp32):
         err = (torch.linalg.norm(zs[at] - zs[0]) / torch.linalg.norm(zs[0])).item()
         say("  autotune %d vs one-wave kernels: rel %.2e" % (at, err))
         assert err <= tol
+
+
+@pytest.mark.parametrize("dims,nl,gl,fp32", [((8, 8, 8), 3, 1, False), ((12, 8, 16), 3, 1, False),
+                                             ((8, 8, 8), 4, 2, False), ((8, 8, 8), 3, 1, True)])
+def test_galerkin_levels_vcycle_matches_restatement(dims, nl, gl, fp32):
+    """Galerkin operators on the levels above the coarsest (ptq2_mg_opts.galerkin_levels),
+    assembled on the GPU by coloured probing into 5x5x5 block stencils: the V-cycle against the
+    restatement whose Galerkin levels are assembled column by column on the CPU (FP64 1e-10,
+    FP32 smoothing 1e-4), on cubic and non-cubic meshes with partial bricks, through a
+    matrix-free level and through a Galerkin level."""
+    mx, my, mz = dims
+    say("case: Galerkin levels %dx%dx%d, %d levels, %d Galerkin, fp32=%s" % (mx, my, mz, nl, gl, fp32))
+    c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 21)
+    mask = M.bc_mask(mx, my, mz, "freeslip")
+    eta = 10.0 ** M.uniform(22, 27 * mx * my * mz, -1.0, 3.0)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(mx, my, mz, c, mask)
+    ctx.set_viscosity(eta)
+    S = ViscousSolver(ctx, nlevels=nl, smooth_its=3, fp32=fp32, galerkin_coarse=True, galerkin_levels=gl, autotune=0)
+    kinds = [k for k, _, _, _ in S.tuning()]
+    assert kinds == [0] * (nl - 1 - gl) + [3] * gl
+    emin, emax = S.eigs()
+    lv = G.hierarchy(mx, my, mz, c, mask, eta, nl, galerkin_levels=gl)
+    V = G.VCycle(lv, emin, emax, 3, galerkin=True)
+    # the eigen-estimates of the Galerkin levels see their assembled operators
+    for l in range(nl - 1 - gl, nl - 1):
+        L = lv[l]
+        v = M.uniform(23 + l, L.n, -1, 1)
+        d = L.diag()
+        for _ in range(60):
+            w = L.matmult(v) / d
+            lam = np.linalg.norm(w) / np.linalg.norm(v)
+            v = w / np.linalg.norm(w)
+        assert 0.8 * lam <= emax[l] / 1.1 <= 1.05 * lam, (l, lam, emax[l])
+    r = M.uniform(24, c.size, -1, 1)
+    r[mask > 0] = 0.0
+    z = S.vcycle(torch.from_numpy(r).to(DEV)).cpu().numpy()
+    zr = V(r)
+    err = np.linalg.norm(z - zr) / np.linalg.norm(zr)
+    say("  rel err %.3e" % err)
+    assert err <= (1e-4 if fp32 else 1e-10)
+    S.destroy()
+    ctx.destroy()

@@ -148,6 +148,52 @@ def test_galerkin_coarse_operator_is_rap():
     assert np.allclose(G.restrict(C.mx, C.my, C.mz, r), (np.stack([G.prolong(C.mx, C.my, C.mz, np.eye(nc)[j]) for j in range(nc)], axis=1)).T @ r)
 
 
+def test_galerkin_level_is_ptap_and_a_5x5x5_stencil():
+    """GalerkinLevel (the restatement of ptq2_mg_opts.galerkin_levels) equals Pi P^T A_f P Pi +
+    (I - Pi) built from an explicit prolongation matrix (Kronecker product of the 1D
+    interpolations, independent of prolong/restrict), couples a node only to nodes within two
+    nodes per axis, and is symmetric to rounding; through a matrix-free level and through a
+    level that is itself Galerkin."""
+    import scipy.sparse as sp
+
+    def p1d(nc):
+        nf = 2 * nc - 1
+        r, c, v = [], [], []
+        for f in range(nf):
+            if f % 2 == 0:
+                r.append(f); c.append(f // 2); v.append(1.0)
+            else:
+                r += [f, f]; c += [f // 2, f // 2 + 1]; v += [0.5, 0.5]
+        return sp.csr_matrix((v, (r, c)), shape=(nf, nc))
+
+    def ptap(F, C, Af):
+        Pk = sp.kron(sp.kron(sp.kron(p1d(2 * C.mz + 1), p1d(2 * C.my + 1)), p1d(2 * C.mx + 1)), sp.identity(3))
+        Pm = sp.diags((F.mask == 0).astype(float)) @ Pk.tocsr() @ sp.diags((C.mask == 0).astype(float))
+        G_ = Pm.T @ Af @ Pm
+        return (G_.toarray() if sp.issparse(G_) else np.asarray(G_)) + np.diag((C.mask > 0).astype(float))
+
+    def check(F, C, Af):
+        want = ptap(F, C, Af)
+        A = C.A.toarray()
+        assert np.abs(A - want).max() <= 1e-12 * np.abs(want).max()
+        assert np.abs(A - A.T).max() <= 1e-12 * np.abs(A).max()
+        NX, NY = 2 * C.mx + 1, 2 * C.my + 1
+        r, c = C.A.nonzero()
+        nr, nc_ = r // 3, c // 3
+        for a, b in ((nr % NX, nc_ % NX), ((nr // NX) % NY, (nc_ // NX) % NY), (nr // (NX * NY), nc_ // (NX * NY))):
+            assert np.abs(a - b).max() <= 2
+
+    P = M.config_sinker(4, n_incl=3, contrast=1e3, bc="freeslip")
+    lv = G.hierarchy(4, 4, 4, P.coords, P.mask, P.eta, 3, galerkin_levels=1)
+    assert type(lv[0]) is G.Level and type(lv[1]) is G.GalerkinLevel and type(lv[2]) is G.Level
+    Af = np.stack([lv[0].matmult(np.eye(lv[0].n)[k]) for k in range(lv[0].n)], axis=1)
+    check(lv[0], lv[1], Af)
+    P = M.config_sinker(8, n_incl=3, contrast=1e3, bc="freeslip")
+    lv = G.hierarchy(8, 8, 8, P.coords, P.mask, P.eta, 4, galerkin_levels=2)
+    assert [type(L) for L in lv] == [G.Level, G.GalerkinLevel, G.GalerkinLevel, G.Level]
+    check(lv[1], lv[2], lv[1].A)
+
+
 def test_stiff_inclusions_match_connected_component_labelling():
     """Inclusions of the rigid-body correction = scipy's 26-connected labelling of the elements
     with a stiff quadrature point (numbered in raster order), and the size cap keeps the largest."""
