@@ -273,13 +273,16 @@ def solve_comparison(m, nlevels, rtol=1e-8, restart=100, max_it=1000, emin_frac=
 
 
 def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1500, restart=256, rigid=True,
-                     emin_frac=0.05, members=None, galerkin_levels=0):
+                     emin_frac=0.05, members=None, galerkin_levels=None, hard_contrast=1e5):
     """Independent buoyancy solves of uneven sizes (viscosity contrasts 1e2..1e6, seeded
     sphere placement, FP32 smoothing), run one after another and then concurrently (one
     stream and host thread per problem).  Host wall time of each variant.  Per problem also
     the rounding floor of the FP64 residual: |A x - A' x| / |b|, A' the same operator
     evaluated through the resident-geometry kernel (another rounding of the same sums); a
-    relative residual much below it is out of reach in FP64."""
+    relative residual much below it is out of reach in FP64.  Problems with a viscosity
+    contrast of at least hard_contrast get Galerkin operators on every level between the fine
+    and the coarsest one and Chebyshev(8) (galerkin_levels given: that many Galerkin levels on
+    every problem, Chebyshev(4))."""
     import torch
 
     import ptq2
@@ -288,7 +291,7 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
 
     contrasts = [10.0 ** (2 + 4 * k / max(1, len(sizes) - 1)) for k in range(len(sizes))]
     keep = list(range(len(sizes))) if members is None else list(members)
-    solvers, rhs, ctxs = [], [], []
+    solvers, rhs, ctxs, config = [], [], [], []
     t = time.perf_counter()
     for k, (m, c) in enumerate(zip(sizes, contrasts)):
         if k not in keep:
@@ -299,8 +302,13 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
         ctx.set_viscosity(P.eta)
         ctxs.append(ctx)
         nl = auto_levels(m)
-        solvers.append(ViscousSolver(ctx, nlevels=nl, smooth_its=4, fp32=True, rigid_deflation=rigid,
-                                     emin_frac=emin_frac, galerkin_levels=min(galerkin_levels, nl - 2)))
+        if galerkin_levels is None:
+            gl, its = (nl - 2, 8) if c >= hard_contrast else (0, 4)
+        else:
+            gl, its = min(galerkin_levels, nl - 2), 4
+        config.append({"levels": nl, "galerkin_levels": gl, "smooth_its": its})
+        solvers.append(ViscousSolver(ctx, nlevels=nl, smooth_its=its, fp32=True, rigid_deflation=rigid,
+                                     emin_frac=emin_frac, galerkin_levels=gl))
         rhs.append(torch.from_numpy(b).to("cuda:0"))
     print("batch set-up %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     for S, b in zip(solvers, rhs):
@@ -321,10 +329,10 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
         floor.append(float("%.3g" % (torch.linalg.norm(y1 - y2).item() / torch.linalg.norm(b).item())))
         del y1, y2
     out = {"workload": "8 buoyancy solves (20 spheres, rho 1.1, free-slip walls), FGMRES(%d) + GMG V-cycle "
-                       "(Chebyshev(4)-Jacobi FP32 on [%g, 1.1] lambda_max)%s%s, rtol %g, at most %d iterations"
-                       % (restart, emin_frac, ", rigid-body deflation of the inclusions" if rigid else "",
-                          ", %d Galerkin levels above the coarsest" % galerkin_levels if galerkin_levels else "",
-                          rtol, max_it),
+                       "(Chebyshev-Jacobi FP32 on [%g, 1.1] lambda_max)%s, rtol %g, at most %d iterations; "
+                       "per problem: levels, Galerkin levels above the coarsest, Chebyshev iterations in 'config'"
+                       % (restart, emin_frac, ", rigid-body deflation of the inclusions" if rigid else "", rtol, max_it),
+           "config": config,
            "sizes": [sizes[k] for k in keep], "contrasts": [float("%.3g" % contrasts[k]) for k in keep],
            "iterations": [i["iterations"] for i in seq], "rel_residual": [i["rel"] for i in seq],
            "fp64_residual_floor": floor,
@@ -400,8 +408,9 @@ def main():
                     help="Chebyshev steps with the update fused into the apply's store (default: the library's)")
     ap.add_argument("--autotune", type=int, default=None, metavar="0|1|2",
                     help="solver launch-shape autotuner (default: the library's, 1); 2 / 3 = row-split / MFMA kernel everywhere")
-    ap.add_argument("--galerkin-levels", type=int, default=0, metavar="K",
-                    help="Galerkin (assembled stencil) operators on the K levels above the coarsest")
+    ap.add_argument("--galerkin-levels", type=int, default=None, metavar="K",
+                    help="Galerkin (assembled stencil) operators on the K levels above the coarsest (solves: "
+                         "default 0; batched solves: default on the problems of contrast >= 1e5 only)")
     ap.add_argument("--batch-only", action="store_true", help="run only the batched solves (--batch)")
     ap.add_argument("--batch-members", default=None, help="comma-separated subset of the 8 batched problems (0..7)")
     ap.add_argument("--solve-only", action="store_true",
@@ -476,7 +485,7 @@ def main():
             extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rg, max_it=args.solve_max_it,
                                           variants=tuple(args.solve_variants.split(",")),
                                           fused_smoother=args.fused_smoother, autotune=args.autotune,
-                                          galerkin_levels=args.galerkin_levels)
+                                          galerkin_levels=args.galerkin_levels or 0)
         print(json.dumps({"solve_only": True, "extra": extra}), flush=True)
         return
 
@@ -542,7 +551,7 @@ def main():
             extra[key] = solve_comparison(args.solve, args.solve_levels, rigid=rigid, max_it=args.solve_max_it,
                                           variants=tuple(args.solve_variants.split(",")),
                                           fused_smoother=args.fused_smoother, autotune=args.autotune,
-                                          galerkin_levels=args.galerkin_levels)
+                                          galerkin_levels=args.galerkin_levels or 0)
         except Exception as exc:  # the apply line above stays valid; report the failure
             extra[key] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     for rigid in variants[args.batch_rigid] if args.batch else []:
