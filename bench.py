@@ -29,6 +29,9 @@ for p in (REPO, os.path.join(REPO, "ptatin-gene_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+# the reference's flop model of the sum-factorized A11 element (src/stokes_operators_avx.c:305-307)
+A11_FLOP_PER_ELEMENT = 17253
 
 
 def algorithmic_bytes_a11(m, value_bytes=8):
@@ -509,6 +512,16 @@ def main():
     achieved = bytes_alg / (ms * 1e-3) / 1e9
     launches = 2
     extra = {}
+    # the same apply against the FP64 arithmetic roof: its algorithmic intensity (17,253 flop per
+    # element over 1,901.6 MB at 128^3: ~19 flop/B) exceeds the machine balance (78.6 TFLOP/s /
+    # 8 TB/s = 9.8 flop/B), so FP64 arithmetic, not HBM, is the roof that binds
+    tflops = A11_FLOP_PER_ELEMENT * P.nel / (ms * 1e-3) / 1e12
+    extra["compute_roofline"] = {"flop_per_element": A11_FLOP_PER_ELEMENT, "achieved_tflops": round(tflops, 2),
+                                 "peak_tflops": FP64_PEAK_TFLOPS, "frac": round(tflops / FP64_PEAK_TFLOPS, 4),
+                                 "intensity_flop_per_byte": round(A11_FLOP_PER_ELEMENT * P.nel / bytes_alg, 2),
+                                 "hbm_frac_at_fp64_peak": round(
+                                     FP64_PEAK_TFLOPS * 1e12 / (A11_FLOP_PER_ELEMENT * P.nel) * bytes_alg / 1e9
+                                     / HBM_PEAK_GBS, 4)}
     if not args.apply_only:
         ms_sg = time_applies(lambda: A.ctx.a11(u, y, _lib.STOREDGEO), args.steps, args.warmup, stream)
         extra["fp64_storedgeo_ms"] = round(ms_sg, 4)
