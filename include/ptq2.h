@@ -41,7 +41,9 @@ enum {
 
 /* flags of the apply entry points */
 #define PTQ2_DEVICE 0x1u  /* vectors are device pointers; the call is stream ordered and
-                             returns before the work completes (ptq2_sync to wait) */
+                             returns before the work completes (ptq2_sync to wait); device
+                             input vectors are not scanned for NaN/Inf (the host path does
+                             that, and ptq2_mg_solve checks its right-hand side's norm) */
 #define PTQ2_FP32 0x2u    /* float vectors; A11 runs the FP32 smoother operator */
 #define PTQ2_MATMULT 0x4u /* MatShell semantics: Dirichlet columns zeroed on input and
                              Dirichlet rows replaced on output */

@@ -67,8 +67,11 @@ typedef struct {
      (ptq2_mg_tuning reports the choice).  The kernels sum in different orders, so two
      hierarchies whose choices differ round differently in the last bits.  0: one wave per
      brick and the mesh's own tile addressing on every level.  2: the three-wave kernel on
-     every level, 3: the matrix-core form on every level (FP32 smoothing), untimed (for
-     tests and comparisons). */
+     every level, 3: the matrix-core form on every level (FP32 smoothing), 4: the one-wave
+     kernel, and R A P through the level above on a Galerkin level below a matrix-free one,
+     untimed (for tests and comparisons).  A Galerkin level below a matrix-free one is timed
+     as its stencil and as R A P (prolongation, the level above's matrix-free apply,
+     restriction), which streams far fewer bytes than the stencil on large levels. */
   int autotune;
   /* k >= 0 (default 0): the k levels above the coarsest carry the Galerkin operator
      Pi R A P Pi + (I - Pi) of the level above them instead of the rediscretised matrix-free
@@ -124,7 +127,13 @@ int ptq2_mg_rigid_info(const ptq2_mg *mg, int *ninclusions, int *ncolumns, int32
 /* the Chebyshev interval [emin, emax] of every smoothed level (nlevels-1 entries each) */
 int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax);
 /* Kernels of the smoother apply (ptq2_mg_tuning) */
-enum { PTQ2_MG_KERNEL_BRICK = 0, PTQ2_MG_KERNEL_SPLIT = 1, PTQ2_MG_KERNEL_MFMA = 2, PTQ2_MG_KERNEL_STENCIL = 3 };
+enum {
+  PTQ2_MG_KERNEL_BRICK = 0,       /* one wave per brick of 4x4x4 elements */
+  PTQ2_MG_KERNEL_SPLIT = 1,       /* three waves per brick, one slab of quadrature rows each */
+  PTQ2_MG_KERNEL_MFMA = 2,        /* the FP32 operator's matrix-core form */
+  PTQ2_MG_KERNEL_STENCIL = 3,     /* Galerkin level: its assembled 5x5x5 block stencil */
+  PTQ2_MG_KERNEL_GALERKIN_MF = 4  /* Galerkin level: R A P through the matrix-free level above */
+};
 /* Launch shape the autotuner chose for the smoother apply of each level l < nlevels - 1:
    kernel (PTQ2_MG_KERNEL_*), tile addressing (1: fast full-tile path, 0: general), and the
    measured ms per apply of the chosen and of the default shape (both 0 when autotune was off

@@ -1102,6 +1102,7 @@ struct ptq2_mg {
   std::vector<char> gal;                // per level: Galerkin stencil operator (galerkin_levels)
   std::vector<double *> gst64;          // its stencil in FP64 (probing, FP64 applies)
   std::vector<float *> gst32;           // and in FP32 (FP32 smoothing)
+  std::vector<void *> gmf_in, gmf_out;  // Galerkin level applied as R A P: vectors of the level above
   std::vector<ptq2::Box> lbox;          // per level: the box of the level's launches (tuned fast32)
   bool prof = false;                    // record per-level events in the eager V-cycle
   std::vector<hipEvent_t> pev;          // 4 per level: entry, before / after the coarser level, exit
@@ -1172,10 +1173,14 @@ int mg_alloc(ptq2_mg *mg, void **p, size_t bytes) {
 template <typename T>
 int level_matmult(ptq2_mg *mg, int l, const T *x, T *y);
 // the level's A11 apply in its tuned launch shape (gather: with the face pass)
+template <typename T>
+hipError_t galerkin_mf(ptq2_mg *mg, int l, const T *x, T *y);
+
 // (the row-split choice is made for the smoothing precision and applies to it only)
 hipError_t level_launch(ptq2_mg *mg, int l, const double *x, double *y, bool gather) {
   ptq2_ctx *c = mg->ctx[l];
   const ptq2::Box &b = mg->lbox[l];
+  if (mg->gal[l] && mg->tune[l].split == 4 && !mg->f32) return galerkin_mf<double>(mg, l, x, y);
   if (mg->gal[l]) {
     hipLaunchKernelGGL(k_stencil_apply<double>, dim3(nblk(c->nnodes)), dim3(TPB), 0, mg->stream, b.NX, b.NY, b.NZ,
                        (const double *)mg->gst64[l], x, y);
@@ -1193,6 +1198,7 @@ hipError_t level_launch(ptq2_mg *mg, int l, const double *x, double *y, bool gat
 hipError_t level_launch(ptq2_mg *mg, int l, const float *x, float *y, bool gather) {
   ptq2_ctx *c = mg->ctx[l];
   const ptq2::Box &b = mg->lbox[l];
+  if (mg->gal[l] && mg->tune[l].split == 4 && mg->f32) return galerkin_mf<float>(mg, l, x, y);
   if (mg->gal[l]) {
     hipLaunchKernelGGL(k_stencil_apply<float>, dim3(nblk(c->nnodes)), dim3(TPB), 0, mg->stream, b.NX, b.NY, b.NZ,
                        (const float *)mg->gst32[l], x, y);
@@ -1203,6 +1209,28 @@ hipError_t level_launch(ptq2_mg *mg, int l, const float *x, float *y, bool gathe
   if (mg->tune[l].split == 2 && mg->f32)
     return ptq2::launch_a11_fp32_mfma(b, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, gather);
   return ptq2::launch_a11_fp32(b, x, c->mask, c->fac32, y, false, mg->stream, (float *)c->facebuf, gather);
+}
+
+// A Galerkin level applied matrix-free: y = Pi R A P Pi x + (I - Pi) x with A the (matrix-free)
+// operator of the level above, in the smoothing precision, through two vectors of that level
+template <typename T>
+hipError_t galerkin_mf(ptq2_mg *mg, int l, const T *x, T *y) {
+  ptq2_ctx *c = mg->ctx[l], *f = mg->ctx[l - 1];
+  T *gp = (T *)mg->gmf_in[l], *ga = (T *)mg->gmf_out[l];
+  const int64_t nf = f->nvel;
+  const int *mc = &mg->m[3 * l], *mf = &mg->m[3 * (l - 1)];
+  const int NXc = 2 * mc[0] + 1, NYc = 2 * mc[1] + 1, NZc = 2 * mc[2] + 1;
+  const int NXf = 2 * mf[0] + 1, NYf = 2 * mf[1] + 1, NZf = 2 * mf[2] + 1;
+  hipLaunchKernelGGL(k_fill<T>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, nf, gp, T(0));
+  hipLaunchKernelGGL(k_prolong_add<T>, dim3(nblk(nf)), dim3(TPB), 0, mg->stream, NXc, NYc, NXf, NYf, NZf, x,
+                     (const uint8_t *)f->mask, gp);
+  hipError_t e = level_launch(mg, l - 1, (const T *)gp, ga, true);
+  if (e == hipSuccess) e = ptq2::launch_bc_rows<T>(f->ndir, f->dir_idx, (const T *)gp, ga, 0, mg->stream);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_restrict<T>, dim3(nblk(c->nvel)), dim3(TPB), 0, mg->stream, NXc, NYc, NZc, NXf, NYf, NZf,
+                     (const T *)ga, (const T *)nullptr, (const uint8_t *)c->mask, y);
+  e = hipGetLastError();
+  return e == hipSuccess ? ptq2::launch_bc_rows<T>(c->ndir, c->dir_idx, x, y, 0, mg->stream) : e;
 }
 
 template <>
@@ -1428,7 +1456,33 @@ int autotune(ptq2_mg *mg) {
   for (int l = 0; l + 1 < mg->nl; l++) {
     ptq2_ctx *c = mg->ctx[l];
     LevelTune &t = mg->tune[l];
-    if (!mg->opt.autotune || !c->facebuf || mg->gal[l]) continue;
+    if (mg->gal[l]) {
+      // a Galerkin level below a matrix-free one: its stencil, or R A P through the level above
+      if (!mg->opt.autotune || mg->gal[l - 1] || mg->opt.autotune == 2 || mg->opt.autotune == 3) continue;
+      const size_t vb = sizeof(T) * (size_t)mg->n[l - 1];
+      if (!mg->gmf_in[l]) MGRC(mg_alloc(mg, &mg->gmf_in[l], vb));
+      if (!mg->gmf_out[l]) MGRC(mg_alloc(mg, &mg->gmf_out[l], vb));
+      if (mg->opt.autotune == 4) {
+        t.split = 4;
+        continue;
+      }
+      const int64_t n = mg->n[l];
+      hipLaunchKernelGGL(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, n, (T *)mg->vx[l], T(1));
+      double t0 = 0.0;
+      MGRC(time_level<T>(mg, l, 2, &t0));
+      const int reps = (int)std::min(50.0, std::max(2.0, 2.0 / std::max(t0, 1e-3)));
+      double ms_st = 0.0, ms_mf = 0.0;
+      t.split = 0;
+      MGRC(time_level<T>(mg, l, reps, &ms_st));
+      t.split = 4;
+      MGRC(time_level<T>(mg, l, reps, &ms_mf));
+      t.ms_default = ms_st;
+      t.split = ms_mf < 0.97 * ms_st ? 4 : 0;
+      t.ms = t.split ? ms_mf : ms_st;
+      continue;
+    }
+    if (!mg->opt.autotune || !c->facebuf) continue;
+    if (mg->opt.autotune == 4) continue;  // the one-wave kernel on the matrix-free levels
     if (mg->opt.autotune >= 2) {  // one kernel everywhere, untimed
       t.split = mg->opt.autotune == 2 ? 1 : (sizeof(T) == 4 ? 2 : 0);
       continue;
@@ -2007,6 +2061,10 @@ int ptq2_mg_destroy(ptq2_mg **pmg) {
     if (p) (void)hipFree(p);
   for (float *p : mg->gst32)
     if (p) (void)hipFree(p);
+  for (void *p : mg->gmf_in)
+    if (p) (void)hipFree(p);
+  for (void *p : mg->gmf_out)
+    if (p) (void)hipFree(p);
   if (mg->exec) (void)hipGraphExecDestroy(mg->exec);
   if (mg->graph) (void)hipGraphDestroy(mg->graph);
   for (size_t l = 0; l < mg->vb.size(); l++) {
@@ -2040,7 +2098,7 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   if (opts) o = *opts;
   else ptq2_mg_default_opts(&o);
   if (o.nlevels < 2 || o.smooth_its < 1 || o.esteig_its < 1 || !(o.emax_frac > 0) || !(o.emin_frac > 0) ||
-      o.autotune < 0 || o.autotune > 3 || o.galerkin_levels < 0 || o.galerkin_levels > o.nlevels - 2 ||
+      o.autotune < 0 || o.autotune > 4 || o.galerkin_levels < 0 || o.galerkin_levels > o.nlevels - 2 ||
       o.coarse_eta_mean < PTQ2_ETA_MEAN_ARITHMETIC || o.coarse_eta_mean > PTQ2_ETA_MEAN_Q1PROJ)
     return PTQ2_ERR_INVALID_ARG;
   if (o.rigid_deflation &&
@@ -2145,6 +2203,8 @@ int ptq2_mg_create(ptq2_mg **out, ptq2_ctx *fine, const ptq2_mg_opts *opts) {
   mg->gal.assign(mg->nl, 0);
   mg->gst64.assign(mg->nl, nullptr);
   mg->gst32.assign(mg->nl, nullptr);
+  mg->gmf_in.assign(mg->nl, nullptr);
+  mg->gmf_out.assign(mg->nl, nullptr);
   for (int l = 0; l + 1 < mg->nl; l++) {
     int rc = PTQ2_OK;
     (void)ptq2_internal_facebuf(mg->ctx[l], &rc);
@@ -2386,7 +2446,9 @@ int ptq2_mg_eigs(const ptq2_mg *mg, double *emin, double *emax) {
 int ptq2_mg_tuning(const ptq2_mg *mg, int *kernel, int *fast_tiles, double *ms, double *ms_default) {
   if (!mg) return PTQ2_ERR_INVALID_ARG;
   for (int l = 0; l + 1 < mg->nl; l++) {
-    if (kernel) kernel[l] = mg->gal[l] ? PTQ2_MG_KERNEL_STENCIL : mg->tune[l].split;
+    if (kernel)
+      kernel[l] = mg->gal[l] ? (mg->tune[l].split == 4 ? PTQ2_MG_KERNEL_GALERKIN_MF : PTQ2_MG_KERNEL_STENCIL)
+                             : mg->tune[l].split;
     if (fast_tiles) fast_tiles[l] = mg->tune[l].fast;
     if (ms) ms[l] = mg->tune[l].ms;
     if (ms_default) ms_default[l] = mg->tune[l].ms_default;

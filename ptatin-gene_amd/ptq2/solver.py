@@ -95,7 +95,7 @@ class ViscousSolver:
             o.fused_smoother = 1 if fused_smoother else 0
         o.galerkin_levels = galerkin_levels
         if autotune is not None:
-            o.autotune = {False: 0, True: 1}.get(autotune, autotune)  # 2 / 3: split / MFMA kernel everywhere
+            o.autotune = {False: 0, True: 1}.get(autotune, autotune)  # 2-4: one forced choice (see ptq2_solver.h)
         o.rigid_deflation = 1 if rigid_deflation else 0
         if rigid_min_contrast is not None:
             o.rigid_min_contrast = rigid_min_contrast
@@ -120,7 +120,7 @@ class ViscousSolver:
         check(_sl().ptq2_mg_eigs(self._h, lo.ctypes.data, hi.ctypes.data), "ptq2_mg_eigs")
         return lo, hi
 
-    KERNELS = ("brick", "split", "mfma", "stencil")  # PTQ2_MG_KERNEL_*
+    KERNELS = ("brick", "split", "mfma", "stencil", "galerkin_mf")  # PTQ2_MG_KERNEL_*
 
     def tuning(self):
         """Per smoothing level: (kernel index into KERNELS, fast tile addressing, ms per apply

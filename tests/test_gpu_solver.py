@@ -382,25 +382,27 @@ def test_autotuner_choices_and_level_times(fp32):
         assert err <= tol
 
 
-@pytest.mark.parametrize("dims,nl,gl,fp32", [((8, 8, 8), 3, 1, False), ((12, 8, 16), 3, 1, False),
-                                             ((8, 8, 8), 4, 2, False), ((8, 8, 8), 3, 1, True)])
-def test_galerkin_levels_vcycle_matches_restatement(dims, nl, gl, fp32):
+@pytest.mark.parametrize("dims,nl,gl,fp32,at", [((8, 8, 8), 3, 1, False, 0), ((12, 8, 16), 3, 1, False, 0),
+                                                ((8, 8, 8), 4, 2, False, 0), ((8, 8, 8), 3, 1, True, 0),
+                                                ((12, 8, 16), 3, 1, False, 4), ((8, 8, 8), 4, 2, True, 4)])
+def test_galerkin_levels_vcycle_matches_restatement(dims, nl, gl, fp32, at):
     """Galerkin operators on the levels above the coarsest (ptq2_mg_opts.galerkin_levels),
     assembled on the GPU by coloured probing into 5x5x5 block stencils: the V-cycle against the
     restatement whose Galerkin levels are assembled column by column on the CPU (FP64 1e-10,
     FP32 smoothing 1e-4), on cubic and non-cubic meshes with partial bricks, through a
-    matrix-free level and through a Galerkin level."""
+    matrix-free level and through a Galerkin level; autotune 4 applies the Galerkin level below
+    the matrix-free one as R A P instead of its stencil."""
     mx, my, mz = dims
-    say("case: Galerkin levels %dx%dx%d, %d levels, %d Galerkin, fp32=%s" % (mx, my, mz, nl, gl, fp32))
+    say("case: Galerkin levels %dx%dx%d, %d levels, %d Galerkin, fp32=%s, autotune %d" % (mx, my, mz, nl, gl, fp32, at))
     c = M.jitter(mx, my, mz, M.q2_uniform(mx, my, mz), 0.1, 1.0 / (2 * max(dims)), 21)
     mask = M.bc_mask(mx, my, mz, "freeslip")
     eta = 10.0 ** M.uniform(22, 27 * mx * my * mz, -1.0, 3.0)
     ctx = ptq2.Context(0)
     ctx.setup_mesh(mx, my, mz, c, mask)
     ctx.set_viscosity(eta)
-    S = ViscousSolver(ctx, nlevels=nl, smooth_its=3, fp32=fp32, galerkin_coarse=True, galerkin_levels=gl, autotune=0)
+    S = ViscousSolver(ctx, nlevels=nl, smooth_its=3, fp32=fp32, galerkin_coarse=True, galerkin_levels=gl, autotune=at)
     kinds = [k for k, _, _, _ in S.tuning()]
-    assert kinds == [0] * (nl - 1 - gl) + [3] * gl
+    assert kinds == [0] * (nl - 1 - gl) + ([4] + [3] * (gl - 1) if at == 4 else [3] * gl)
     emin, emax = S.eigs()
     lv = G.hierarchy(mx, my, mz, c, mask, eta, nl, galerkin_levels=gl)
     V = G.VCycle(lv, emin, emax, 3, galerkin=True)
