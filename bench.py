@@ -339,6 +339,7 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
            "config": config,
            "sizes": [sizes[k] for k in keep], "contrasts": [float("%.3g" % contrasts[k]) for k in keep],
            "iterations": [i["iterations"] for i in seq], "rel_residual": [i["rel"] for i in seq],
+           "stopped_at_floor": [i["stagnated"] for i in seq],
            "fp64_residual_floor": floor,
            "batched_iterations_equal": [a["iterations"] == b[1]["iterations"] for a, b in zip(seq, bat)],
            "sequential_s": round(t_seq, 4), "concurrent_s": round(t_bat, 4),

@@ -161,6 +161,13 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
                   int *iterations, double *final_rnorm, double *rnorm0);
 
 /* residual history of the last solve (one norm per iteration, up to cap entries) */
+/* 1 if the last ptq2_mg_solve stopped at the rounding floor: a restart cycle whose residual
+   estimate reached rtol left a true residual (b - A x in FP64) above 1.5 rtol that had
+   fallen by less than 10% since the cycle began, so rtol lies below what the FP64 residual of
+   this problem resolves (stiff
+   inclusions: their element sums cancel only to ~eps times the inclusion viscosity).  The
+   solve then returns with that true residual in final_rnorm instead of running to max_it. */
+int ptq2_mg_stagnated(const ptq2_mg *mg);
 int ptq2_mg_history(const ptq2_mg *mg, double *hist, int cap, int *n);
 
 /* message of the last failing call on this hierarchy ("" when none) */

@@ -1129,6 +1129,7 @@ struct ptq2_mg {
   const double **Vptr = nullptr;  // device table of the finished basis vectors (Vk or V32)
   const double **Zptr = nullptr;
   std::vector<double> hist;
+  bool stagnated = false;  // the last solve stopped at the rounding floor
   std::string last;
   void *fine_stream0 = nullptr;  // the fine context's stream before the hierarchy was built
   uint64_t fine_gen = 0;         // the fine context's mesh/viscosity generation at set-up
@@ -2592,6 +2593,14 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
     return std::fabs(g[j + 1]);
   };
   const double **Vp = mg->Vptr;
+  // Stop at the rounding floor: a cycle whose estimate reached the target while its true
+  // residual fell by less than 10%, and stays at least 1.5x the target, means the FP64
+  // residual cannot resolve the target (the estimate and the true residual part once the
+  // update falls below the rounding of A x).  A claim missed by a few percent near the target,
+  // or one after which the true residual still falls, lets the solve go on.
+  bool prev_claimed = false;
+  double prev_beta = 0.0;
+  mg->stagnated = false;
   while (it < max_it) {
     // r = b - A x, the unnormalised first candidate
     MGRC(ptq2_a11_apply(fc, x, mg->w, PTQ2_DEVICE | PTQ2_MATMULT));
@@ -2602,6 +2611,11 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
       rn = beta;
       break;
     }
+    if (prev_claimed && beta > 1.5 * target && beta > 0.9 * prev_beta) {
+      mg->stagnated = true;
+      break;
+    }
+    prev_beta = beta;
     std::fill(g.begin(), g.end(), 0.0);
     std::fill(H.begin(), H.end(), 0.0);
     g[0] = beta;
@@ -2698,6 +2712,7 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
         k = j + 1;
       }
     }
+    prev_claimed = rn <= target;
     // x += Z y, with H(0:k,0:k) y = g
     for (int i = k - 1; i >= 0; i--) {
       double s = g[i];
@@ -2722,6 +2737,8 @@ int ptq2_mg_solve(ptq2_mg *mg, const double *b, double *x, double rtol, int max_
 }
 
 const char *ptq2_mg_last_error(const ptq2_mg *mg) { return mg ? mg->last.c_str() : ""; }
+
+int ptq2_mg_stagnated(const ptq2_mg *mg) { return mg && mg->stagnated ? 1 : 0; }
 
 int ptq2_mg_history(const ptq2_mg *mg, double *hist, int cap, int *n) {
   if (!mg || !n) return PTQ2_ERR_INVALID_ARG;

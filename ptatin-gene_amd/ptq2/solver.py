@@ -43,6 +43,7 @@ _SOLVER_SIGS = {
                                      ctypes.POINTER(ctypes.c_double)]),
     "ptq2_mg_history": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
     "ptq2_mg_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "ptq2_mg_stagnated": (ctypes.c_int, [ctypes.c_void_p]),
 }
 _bound = False
 
@@ -185,7 +186,8 @@ class ViscousSolver:
         _mgcheck(_sl().ptq2_mg_solve(self._h, b.data_ptr(), x.data_ptr(), rtol, max_it, restart, ctypes.byref(it),
                                      ctypes.byref(fr), ctypes.byref(r0)), "ptq2_mg_solve", self._h)
         return x, {"iterations": it.value, "final_rnorm": fr.value, "rnorm0": r0.value,
-                   "rel": fr.value / r0.value if r0.value else 0.0}
+                   "rel": fr.value / r0.value if r0.value else 0.0,
+                   "stagnated": bool(_sl().ptq2_mg_stagnated(self._h))}
 
     def history(self):
         n = ctypes.c_int()
