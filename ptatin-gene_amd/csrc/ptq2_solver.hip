@@ -94,26 +94,16 @@ __global__ void k_cheb_first(int64_t n, const T *__restrict__ b, const T *__rest
 // face_only: the apply's store already updated the Dirichlet dofs and every dof that no face
 // slot adds to (store_tile_own<true>); only the lower-face dofs are left.
 template <typename T>
-__global__ __launch_bounds__(64) void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b,
-                                                   const T *__restrict__ y, const T *__restrict__ dinv,
-                                                   const T *__restrict__ x, const T *__restrict__ pkm1,
-                                                   T *__restrict__ out, const T *__restrict__ face,
-                                                   const uint8_t *__restrict__ mask, T cr, T ck, T cx, int face_only) {
-  // one wave per node row (y, z): the row's 3 NX dofs are contiguous, so every access is
-  // coalesced and the brick/face indices come from the wave-uniform row plus d / 3
+__global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T *__restrict__ y,
+                             const T *__restrict__ dinv, const T *__restrict__ x, const T *__restrict__ pkm1,
+                             T *__restrict__ out, const T *__restrict__ face, const uint8_t *__restrict__ mask, T cr,
+                             T ck, T cx, int face_only) {
   const ptq2::ChebOut<T> co{b, dinv, x, pkm1, out, mask, cr, ck, cx};
   constexpr int TN = ptq2::TN, F2 = TN * TN, UP = 2 * ptq2::BE, NB = 2 * ptq2::BE;
+  const unsigned NX = (unsigned)bx.NX, NXY = (unsigned)bx.NX * (unsigned)bx.NY;
   const int64_t sx = ptq2::FACEVALS, sy = (int64_t)bx.nbx * ptq2::FACEVALS,
                 sz = (int64_t)bx.nbx * bx.nby * ptq2::FACEVALS;
-  const int row = blockIdx.x, Yn = row % bx.NY, Z = row / bx.NY;
-  const int byi = min(Yn / NB, bx.nby - 1), bzi = min(Z / NB, bx.nbz - 1);
-  const int ly = Yn - NB * byi, lz = Z - NB * bzi;
-  const bool ny = ly == 0 && byi > 0, nz = lz == 0 && bzi > 0;
-  const int64_t r0 = 3 * (int64_t)bx.NX * row;
-  (void)n;
-  for (int d = threadIdx.x; d < 3 * bx.NX; d += 64) {
-    const int64_t i = r0 + d;
-    const int X = d / 3, c = d - 3 * X;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     T yv;
     if (mask && mask[i]) {
       if (face_only) continue;
@@ -121,8 +111,12 @@ __global__ __launch_bounds__(64) void k_cheb_faces(Box bx, int64_t n, const T *_
     } else {
       yv = y[i];
       if (face) {
-        const int bxi = min(X / NB, bx.nbx - 1), lx = X - NB * bxi;
-        const bool nx = lx == 0 && bxi > 0;
+        const unsigned nd = (unsigned)(i / 3), c = (unsigned)(i - 3 * (int64_t)nd);
+        const unsigned Z = nd / NXY, rem = nd - Z * NXY, Yn = rem / NX, X = rem - Yn * NX;
+        const int bxi = min((int)(X / NB), bx.nbx - 1), byi = min((int)(Yn / NB), bx.nby - 1),
+                  bzi = min((int)(Z / NB), bx.nbz - 1);
+        const int lx = (int)X - NB * bxi, ly = (int)Yn - NB * byi, lz = (int)Z - NB * bzi;
+        const bool nx = lx == 0 && bxi > 0, ny = ly == 0 && byi > 0, nz = lz == 0 && bzi > 0;
         if (face_only && !(nx || ny || nz)) continue;
         if (nx || ny || nz) {
           const T *fb = face + ((int64_t)bxi + (int64_t)bx.nbx * (byi + (int64_t)bx.nby * bzi)) * ptq2::FACEVALS;
@@ -1309,8 +1303,7 @@ int chebyshev(ptq2_mg *mg, int l, bool zero_guess) {
       MGRC(level_apply_cheb<T>(mg, l, xin, y, co));
     else
       MGRC(level_apply_owned<T>(mg, l, xin, y));
-    hipLaunchKernelGGL(k_cheb_faces<T>, dim3(lc->box.NY * lc->box.NZ), dim3(64), 0, mg->stream, lc->box, n,
-                       (const T *)b,
+    hipLaunchKernelGGL(k_cheb_faces<T>, dim3(nblk(n)), dim3(TPB), 0, mg->stream, lc->box, n, (const T *)b,
                        (const T *)y, (const T *)dinv, xin, pkm1, out, face,
                        (const uint8_t *)lc->mask, (T)cr, (T)ck, (T)cx, fz ? 1 : 0);
     return PTQ2_OK;
