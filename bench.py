@@ -312,7 +312,8 @@ def batch_comparison(sizes=(24, 32, 40, 48, 56, 64, 64, 80), rtol=1e-8, max_it=1
         solvers.append(ViscousSolver(ctx, nlevels=nl, smooth_its=its, fp32=True, rigid_deflation=rigid,
                                      emin_frac=emin_frac, galerkin_levels=gl))
         config.append({"levels": nl, "galerkin_levels": gl, "smooth_its": its,
-                       "kernels": [ViscousSolver.KERNELS[kk] for kk, _, _, _ in solvers[-1].tuning()]})
+                       "kernels": [ViscousSolver.KERNELS[kk] for kk, _, _, _ in solvers[-1].tuning()],
+                       "kernel_ms": [[round(a, 4), round(d, 4)] for _, _, a, d in solvers[-1].tuning()]})
         rhs.append(torch.from_numpy(b).to("cuda:0"))
     print("batch set-up %.1f s" % (time.perf_counter() - t), file=sys.stderr, flush=True)
     for S, b in zip(solvers, rhs):

@@ -5,6 +5,7 @@ import pytest
 from oracle import mg_oracle as G
 from oracle import oracle as O
 import ptq2
+from ptq2 import _lib
 from ptq2 import mesh as M
 from ptq2.solver import ViscousSolver
 
@@ -245,6 +246,11 @@ def test_rigid_deflation_options_and_cap():
     for bad in ({"rigid_max_inclusions": 0}, {"rigid_max_inclusions": 129}, {"rigid_min_contrast": 0.5}):
         with pytest.raises(ptq2.PTQ2Error):
             ViscousSolver(ctx, nlevels=3, fp32=True, rigid_deflation=True, **bad)
+    # the other options' ranges: Galerkin levels 0 .. nlevels-2, autotune 0 .. 4
+    for bad in ({"galerkin_levels": 2}, {"galerkin_levels": -1}, {"autotune": 5}, {"autotune": -1}):
+        with pytest.raises(ptq2.PTQ2Error) as ei:
+            ViscousSolver(ctx, nlevels=3, fp32=True, **bad)
+        assert ei.value.code == _lib.ERR_INVALID_ARG
 
 
 def test_rigid_deflation_speeds_up_high_contrast_solve():
