@@ -99,10 +99,11 @@ __device__ __forceinline__ void a11_qp64(const double ur0[3][3], const double ur
 // On entry ca holds column 0 of the coordinate tile; on exit it holds it again (for
 // the next row), its reads in flight behind the stress work.  BLK adds the pressure and
 // continuity terms of the full block.
-template <bool BLK>
+template <bool BLK, bool SLAB = false>
 __device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, int base, const Coef1 &cb,
                                           const Coef1 &cc, double wbc, const double e3[3], double ca[9],
-                                          double cb9[9], double Y[3][3][3][3], int row, BlockState *bs) {
+                                          double cb9[9], double Y[3][3][3][3], int row, BlockState *bs,
+                                          double (*RB)[3][3] = nullptr, double (*RD)[3][3] = nullptr) {
   double A0[3][3], A1[3][3], A2[3][3], s0, s1, s2;
   double np[3][3], wd[3] = {0.0, 0.0, 0.0};
   {
@@ -133,7 +134,10 @@ __device__ __forceinline__ void a11_row64(const double *Xs, const double *Us, in
   a11_qp64<1, BLK>(ur0, ur1, ur2, A1, s1, wbc, np[1], wd[1], 3 * row + 1, bs, P0, P1, P2);
   a11_qp64<2, BLK>(ur0, ur1, ur2, A2, s2, wbc, np[2], wd[2], 3 * row + 2, bs, P0, P1, P2);
   row_fence();
-  acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
+  if constexpr (SLAB)
+    acc_row_slab_rt(cb, P0, P1, P2, RB, RD);
+  else
+    acc_row_elem_rt(cb, cc, P0, P1, P2, Y);
   row_fence();
 }
 
@@ -210,12 +214,37 @@ __device__ __forceinline__ void a11_fp64_brick(const Box &b, const BrickId &br, 
   if (active) {
     double ca[9], cb9[9];
     ld_col(Xs, base, 0, ca);
+    if constexpr (!BLK) {
+      // slab form of the y/z transposes (the full block keeps the per-row form: with its
+      // pressure state the slab sums would spill)
 #pragma nounroll
-    for (int row = 0; row < 9; row++) {
-      const int qb = row % 3, qc = row / 3;
-      double e3[3] = {en[0], en[1], en[2]};
-      if (row < 8) load3(ep, row + 1, en);
-      a11_row64<BLK>(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y, row, &bs);
+      for (int qc = 0; qc < 3; qc++) {
+        double RB[3][3][3], RD[3][3][3];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+          for (int j = 0; j < 3; j++)
+#pragma unroll
+            for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0;
+#pragma nounroll
+        for (int qb = 0; qb < 3; qb++) {
+          const int row = qb + 3 * qc;
+          double e3[3] = {en[0], en[1], en[2]};
+          if (row < 8) load3(ep, row + 1, en);
+          a11_row64<BLK, true>(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y, row, &bs,
+                               RB, RD);
+        }
+        acc_slab_rt(coef1(qc), RB, RD, Y);
+        row_fence();
+      }
+    } else {
+#pragma nounroll
+      for (int row = 0; row < 9; row++) {
+        const int qb = row % 3, qc = row / 3;
+        double e3[3] = {en[0], en[1], en[2]};
+        if (row < 8) load3(ep, row + 1, en);
+        a11_row64<BLK>(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y, row, &bs);
+      }
     }
     if constexpr (BLK) {
       double *o = yp + 4 * e;
@@ -467,15 +496,23 @@ __global__ __launch_bounds__(3 * BEL) void k_a11_fp64_split(Box b, const double 
         for (int i = 0; i < 3; i++) Y[c][k][j][i] = 0.0;
   BlockState bs;
   if (active) {
-    double ca[9], cb9[9];
+    double ca[9], cb9[9], RB[3][3][3], RD[3][3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) RB[c][j][i] = RD[c][j][i] = 0.0;
     ld_col(Xs, base, 0, ca);
 #pragma nounroll
     for (int row = 3 * w; row < 3 * w + 3; row++) {
-      const int qb = row % 3, qc = row / 3;
+      const int qb = row % 3;
       double e3[3] = {en[0], en[1], en[2]};
       if (row + 1 < 3 * w + 3) load3(ep, row + 1, en);
-      a11_row64<false>(Xs, Us, base, coef1(qb), coef1(qc), c_gw[qb] * c_gw[qc], e3, ca, cb9, Y, row, &bs);
+      a11_row64<false, true>(Xs, Us, base, coef1(qb), coef1(w), c_gw[qb] * c_gw[w], e3, ca, cb9, Y, row, &bs, RB,
+                             RD);
     }
+    acc_slab_rt(coef1(w), RB, RD, Y);  // the wave's slab (qc = w) into its outputs
   }
   __syncthreads();
   if (w == 0) zero_tile(Xs, lane);
