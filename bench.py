@@ -433,6 +433,8 @@ def main():
                     help="time the A11 of the 8 batched-solve problems: 8 single launches vs one batched launch")
     ap.add_argument("--batch-apply-only", action="store_true",
                     help="run only the batched-apply comparison (the command of its profiler runs)")
+    ap.add_argument("--body-force-only", action="store_true",
+                    help="run only the body-force timing (the command of its profiler runs)")
     ap.add_argument("--block", type=int, default=1, metavar="0|1",
                     help="time the full Stokes block (FP64 and FP32) on the contrast-1e6 sinker (default 1)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -471,6 +473,11 @@ def main():
     if args.batch_apply_only:
         extra = {"batch_apply": batch_apply_comparison(args.steps, args.warmup, torch.cuda.current_stream())}
         print(json.dumps({"batch_apply_only": True, "extra": extra}), flush=True)
+        return
+
+    if args.body_force_only:
+        extra = {"body_force": body_force_timing(args.size, args.steps, args.warmup, torch.cuda.current_stream())}
+        print(json.dumps({"body_force_only": True, "extra": extra}), flush=True)
         return
 
     if args.batch_only:

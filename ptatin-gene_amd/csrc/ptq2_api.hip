@@ -641,7 +641,9 @@ int ptq2_body_force(ptq2_ctx *c, const double *fu, double *y, unsigned flags) {
     df = c->sp;
     dy = (double *)c->sy;
   }
-  HIPCHK(c, ptq2::launch_body_force(c->box, c->coords, df, dy, add, c->stream));
+  void *fb = ptq2_internal_facebuf(c, &rc);
+  if (rc) return rc;
+  HIPCHK(c, ptq2::launch_body_force(c->box, c->coords, df, dy, add, (double *)fb, c->stream));
   if (mm) HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, (const double *)nullptr, dy, 1, c->stream));
   if (!dev) {
     HIPCHK(c, hipMemcpyAsync(y, dy, sizeof(double) * (size_t)c->nvel, hipMemcpyDeviceToHost, c->stream));
@@ -799,7 +801,7 @@ int ptq2_stokes_residual(ptq2_ctx *c, const double *u, const double *p, const do
   // Fu = A11 u_loc + A12 p, Fp = A21 u_loc - int N^p fp (one element pass), then Fu -= Be
   HIPCHK(c, ptq2::launch_stokes_fp64(c->box, c->coords, ul, dp, nullptr, c->eta_bm, dFu, dFp, dfp, false, c->stream,
                                      (double *)fb));
-  HIPCHK(c, ptq2::launch_body_force(c->box, c->coords, dfu, dFu, true, c->stream, -1.0));
+  HIPCHK(c, ptq2::launch_body_force(c->box, c->coords, dfu, dFu, true, (double *)fb, c->stream, -1.0));
   // Dirichlet rows: Fu_D = u_D - phi_D (BCListResidualDirichlet, src/dmda_bcs.c:614-648)
   HIPCHK(c, ptq2::launch_bc_rows<double>(c->ndir, c->dir_idx, du, dFu, 3, c->stream, dphi));
   if (!dev) {

@@ -78,8 +78,9 @@ hipError_t launch_a21(const Box &b, const double *coords, const double *u, const
 hipError_t launch_a12(const Box &b, const double *coords, const double *p, double *y, bool add, hipStream_t s);
 hipError_t launch_diag(const Box &b, const double *coords, const double *eta_bm, double *d, bool add, hipStream_t s);
 // y (+)= the body-force load of fu (3 per quadrature point, element-major)
+// one launch with owner-writes output into facebuf (nbricks * FACEVALS doubles) plus the face pass
 hipError_t launch_body_force(const Box &b, const double *coords, const double *fu, double *y, bool add,
-                             hipStream_t s, double scale = 1.0);
+                             double *facebuf, hipStream_t s, double scale = 1.0);
 template <typename T>
 hipError_t launch_bc_rows(int64_t n, const int64_t *idx, const T *x, T *y, int mode, hipStream_t s,
                           const T *v = nullptr);

@@ -1091,84 +1091,130 @@ __global__ __launch_bounds__(BEL) void k_a12(Box b, int color, const double *__r
 
 // ---------------------------------------------------------------- body force
 // y[3n+d] += sum_e sum_q w_q det J_q N_n(x_q) fu[e][q][d]: the Be term of FormFunctionLocal_U
-// (src/stokes_form_function.c:219-224), the momentum load of buoyancy problems.  Per element the
-// 27 point values w det fu, then the transposed Q2 interpolation one axis at a time.
+// (src/stokes_form_function.c:219-224), the momentum load of buoyancy problems.  One launch over
+// all bricks with the owner-writes store (face slots, then k_gather_faces).  Per element the 27
+// weights w det J come from the coordinate tile; its LDS then becomes the output tile.  Then, one
+// z-slab of quadrature points (qc) at a time, each lane reads its element's 27 values of the slab
+// (consecutive in fu, 16-byte loads), scales them by w det, applies the transposed Q2
+// interpolation along x and y and adds the slab's z factor times that into the output tile.
+// With nothing but the weights and two slabs held, two bricks run per SIMD, so one brick's loads
+// overlap the other's work.
 
+// the lane's element's 27 values of slab qc (zeros outside the box, never used): 13 16-byte loads
+// and one 8-byte load, the single one first or last as the slab's address is or is not 16-byte
+// aligned
+__device__ __forceinline__ void bf_slab_load(const double *__restrict__ fe, bool active, int qc, double r[27]) {
+  const double *p = fe + 27 * qc;
+  const bool al = ((uintptr_t)p & 15) == 0;
+  const double2 *__restrict__ p4 = (const double2 *)(al ? p : p + 1);
+  const double *__restrict__ p1 = al ? p + 26 : p;
+  double2 v[13];
+  double w = 0.0;
+#pragma unroll
+  for (int k = 0; k < 13; k++) v[k] = active ? p4[k] : make_double2(0.0, 0.0);
+  if (active) w = *p1;
+#pragma unroll
+  for (int k = 0; k < 27; k++) {
+    const double a = k < 26 ? (k & 1 ? v[k >> 1].y : v[k >> 1].x) : w;
+    const double m = k == 0 ? w : ((k - 1) & 1 ? v[(k - 1) >> 1].y : v[(k - 1) >> 1].x);
+    r[k] = al ? a : m;
+  }
+}
+
+// geometry weight w det J (times sc) of quadrature point (qa, qb, qc)
 template <int qa, int qb, int qc>
-__device__ __forceinline__ void bf_qp(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
-                                      const double *__restrict__ fe, double scale, double F[3][27]) {
+__device__ __forceinline__ void bf_weight(const double xr0[3][3], const double xr1[3][3], const double xr2[3][3],
+                                          double sc, double wd[27]) {
   double J[3][3], A[3][3];
   qp_grad<qa>(xr0, xr1, xr2, J);
   const double det = adjugate(J, A);
-  constexpr int q = qa + 3 * qb + 9 * qc;
   constexpr double w = gw1(qa) * gw1(qb) * gw1(qc);
-  const double s = scale * (w * det);
-#pragma unroll
-  for (int d = 0; d < 3; d++) F[d][q] = s * fe[3 * q + d];
+  wd[qa + 3 * qb + 9 * qc] = sc * (w * det);
 }
 
 template <int qb, int qc>
-__device__ __forceinline__ void bf_row(const double *Xs, int base, const double *__restrict__ fe, double sc,
-                                       double F[3][27]) {
+__device__ __forceinline__ void bf_row(const double *Xs, int base, double sc, double wd[27]) {
   double xr0[3][3], xr1[3][3], xr2[3][3];
   row_data<qb, qc>(Xs, base, xr0, xr1, xr2);
-  bf_qp<0, qb, qc>(xr0, xr1, xr2, fe, sc, F);
-  bf_qp<1, qb, qc>(xr0, xr1, xr2, fe, sc, F);
-  bf_qp<2, qb, qc>(xr0, xr1, xr2, fe, sc, F);
+  bf_weight<0, qb, qc>(xr0, xr1, xr2, sc, wd);
+  bf_weight<1, qb, qc>(xr0, xr1, xr2, sc, wd);
+  bf_weight<2, qb, qc>(xr0, xr1, xr2, sc, wd);
 }
 
-__global__ __launch_bounds__(BEL) void k_body_force(Box b, int color, const double *__restrict__ coords,
+// slab qc into the output tile: F = w det fu at the slab's 9 points, B^T along x, then y, then
+// B(qc, k) into node layer k (ds_add_f64: the 64 addresses of one instruction are distinct, so
+// the order of the sums is the program order)
+template <int qc>
+__device__ __forceinline__ void bf_slab(double *Ys, int base, const double r[27], const double wd[27]) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    double T1[3][3];  // [qb][i]
+#pragma unroll
+    for (int qb = 0; qb < 3; qb++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        double a = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int qa = 0; qa < 3; qa++) {
+          if (!nzB(qa, i)) continue;
+          const double f = wd[qa + 3 * qb + 9 * qc] * r[3 * (qa + 3 * qb) + d];
+          a = first ? gB(qa, i) * f : fma(gB(qa, i), f, a);
+          first = false;
+        }
+        T1[qb][i] = a;
+      }
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        double a = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int qb = 0; qb < 3; qb++) {
+          if (!nzB(qb, j)) continue;
+          a = first ? gB(qb, j) * T1[qb][i] : fma(gB(qb, j), T1[qb][i], a);
+          first = false;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+          if (nzB(qc, k)) lds_add(&Ys[base + toff(d, i, j, k)], gB(qc, k) * a);
+      }
+  }
+}
+
+__global__ __launch_bounds__(BEL) void k_body_force(Box b, const double *__restrict__ coords,
                                                     const double *__restrict__ fu, double *__restrict__ y,
-                                                    int add_mode, double sc) {
+                                                    int add_mode, double sc, double *__restrict__ facebuf) {
   __shared__ double Xs[3 * TCS];
   const int lane = threadIdx.x;
-  const BrickId br = brick_of(b, color, blockIdx.x);
+  const BrickId br = brick_xcd(b, blockIdx.x);
   const Tile t = tile_of(b, br);
+  const bool active = lane_active(t, lane);
+  const double *fe = fu + 81 * (active ? lane_element(b, t, lane) : 0);
+  double r0[27], r1[27];
+  bf_slab_load(fe, active, 0, r0);
   load_tile(b, t, br, coords, (const uint8_t *)nullptr, Xs, lane);
   __syncthreads();
   const int base = lane_base(lane);
-  const bool active = lane_active(t, lane);
-  double Y[3][3][3][3];
+  double wd[27];
   if (active) {
-    const double *fe = fu + 81 * lane_element(b, t, lane);
-    double F[3][27];
-    bf_row<0, 0>(Xs, base, fe, sc, F); bf_row<1, 0>(Xs, base, fe, sc, F); bf_row<2, 0>(Xs, base, fe, sc, F);
-    bf_row<0, 1>(Xs, base, fe, sc, F); bf_row<1, 1>(Xs, base, fe, sc, F); bf_row<2, 1>(Xs, base, fe, sc, F);
-    bf_row<0, 2>(Xs, base, fe, sc, F); bf_row<1, 2>(Xs, base, fe, sc, F); bf_row<2, 2>(Xs, base, fe, sc, F);
-    // B^T along x, then y, then z: Y[d][k][j][i] = sum B(qa,i) B(qb,j) B(qc,k) F[d][qa+3qb+9qc]
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      double T1[3][3][3], T2[3][3][3];  // T1[qc][qb][i], T2[qc][j][i]
-#pragma unroll
-      for (int qc = 0; qc < 3; qc++)
-#pragma unroll
-        for (int qb = 0; qb < 3; qb++)
-#pragma unroll
-          for (int i = 0; i < 3; i++)
-            T1[qc][qb][i] = gB(0, i) * F[d][9 * qc + 3 * qb] + gB(1, i) * F[d][9 * qc + 3 * qb + 1] +
-                            gB(2, i) * F[d][9 * qc + 3 * qb + 2];
-#pragma unroll
-      for (int qc = 0; qc < 3; qc++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++)
-            T2[qc][j][i] = gB(0, j) * T1[qc][0][i] + gB(1, j) * T1[qc][1][i] + gB(2, j) * T1[qc][2][i];
-#pragma unroll
-      for (int k = 0; k < 3; k++)
-#pragma unroll
-        for (int j = 0; j < 3; j++)
-#pragma unroll
-          for (int i = 0; i < 3; i++)
-            Y[d][k][j][i] = gB(0, k) * T2[0][j][i] + gB(1, k) * T2[1][j][i] + gB(2, k) * T2[2][j][i];
-    }
+    bf_row<0, 0>(Xs, base, sc, wd); row_fence(); bf_row<1, 0>(Xs, base, sc, wd); row_fence();
+    bf_row<2, 0>(Xs, base, sc, wd); row_fence(); bf_row<0, 1>(Xs, base, sc, wd); row_fence();
+    bf_row<1, 1>(Xs, base, sc, wd); row_fence(); bf_row<2, 1>(Xs, base, sc, wd); row_fence();
+    bf_row<0, 2>(Xs, base, sc, wd); row_fence(); bf_row<1, 2>(Xs, base, sc, wd); row_fence();
+    bf_row<2, 2>(Xs, base, sc, wd);
   }
-  __syncthreads();
+  bf_slab_load(fe, active, 1, r1);
+  __syncthreads();  // the coordinate tile is read: its LDS becomes the output tile
   zero_tile(Xs, lane);
   __syncthreads();
-  if (active) emit_element(Xs, base, Y);
+  if (active) bf_slab<0>(Xs, base, r0, wd);
+  bf_slab_load(fe, active, 2, r0);
+  if (active) bf_slab<1>(Xs, base, r1, wd);
+  if (active) bf_slab<2>(Xs, base, r0, wd);
   __syncthreads();
-  store_tile(b, br, t, Xs, y, add_mode != 0, lane);
+  store_tile_own(b, br, t, Xs, y, facebuf, add_mode != 0, lane);
 }
 
 // ---------------------------------------------------------------- diagonal
@@ -1561,13 +1607,10 @@ hipError_t launch_a12(const Box &b, const double *coords, const double *p, doubl
 }
 
 hipError_t launch_body_force(const Box &b, const double *coords, const double *fu, double *y, bool add,
-                             hipStream_t s, double scale) {
-  for (int c = 0; c < 8; c++) {
-    int nb = nbricks_color(b, c);
-    if (nb == 0) continue;
-    hipLaunchKernelGGL(k_body_force, dim3(nb), dim3(BEL), 0, s, b, c, coords, fu, y, add ? 1 : 0, scale);
-  }
-  return hipGetLastError();
+                             double *facebuf, hipStream_t s, double scale) {
+  hipLaunchKernelGGL(k_body_force, dim3(nbricks(b)), dim3(BEL), 0, s, b, coords, fu, y, add ? 1 : 0, scale, facebuf);
+  hipError_t e = hipGetLastError();
+  return e != hipSuccess ? e : launch_gather(b, (const double *)facebuf, y, s);
 }
 
 hipError_t launch_diag(const Box &b, const double *coords, const double *eta_bm, double *d, bool add, hipStream_t s) {
