@@ -132,6 +132,33 @@ def block_comparison(m, steps, warmup, stream):
     return out
 
 
+def operator_timings(ctx, u, P, steps, warmup, stream):
+    """The other single-operator entry points on the bench's 128^3 context: A12 (gradient), A21
+    (divergence) and the A11 diagonal, each with its algorithmic bytes (every array once, int32
+    connectivity per element node, the convention of the A11 count) and the resulting rate."""
+    import torch
+
+    import ptq2
+
+    ne, n3 = P.nel, P.nvel
+    conn = 4 * 27 * ne
+    p = torch.from_numpy(ptq2.mesh.uniform(3, 4 * ne, -1.0, 1.0)).to(u.device)
+    yu = torch.empty_like(u)
+    yp = torch.empty_like(p)
+    d = torch.empty_like(u)
+    out = {}
+    for name, fn, nbytes in (
+            ("a12", lambda: ctx.a12(p, yu), 8 * 4 * ne + 8 * n3 + 8 * n3 + conn),
+            ("a21", lambda: ctx.a21(u, yp), 8 * n3 + 8 * n3 + 8 * 4 * ne + conn),
+            ("diag", lambda: ctx.a11_diag(d), 8 * n3 + 8 * 27 * ne + 8 * n3 + conn)):
+        ms = time_applies(fn, steps, warmup, stream)
+        out[name] = {"ms": round(ms, 4), "algorithmic_gb": round(nbytes / 1e9, 3),
+                     "gbs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+                     "hbm_frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    del p, yu, yp, d
+    return out
+
+
 def body_force_timing(m, steps, warmup, stream):
     """The body-force load (ptq2_body_force) on the 128^3 mesh from uniform random quadrature
     values: time per call and its bandwidth over the algorithmic bytes (fu 81 x 8 B per element,
@@ -433,6 +460,8 @@ def main():
                     help="time the A11 of the 8 batched-solve problems: 8 single launches vs one batched launch")
     ap.add_argument("--batch-apply-only", action="store_true",
                     help="run only the batched-apply comparison (the command of its profiler runs)")
+    ap.add_argument("--operators-only", action="store_true",
+                    help="run only the A12 / A21 / A11-diagonal timings (the command of their profiler runs)")
     ap.add_argument("--body-force-only", action="store_true",
                     help="run only the body-force timing (the command of its profiler runs)")
     ap.add_argument("--block", type=int, default=1, metavar="0|1",
@@ -516,6 +545,10 @@ def main():
 
     if args.settle > 0:
         settle_clocks(lambda: A.ctx.a11(u, y, 0), args.settle)
+    if args.operators_only:
+        extra = {"operators": operator_timings(A.ctx, u, P, args.steps, args.warmup, stream)}
+        print(json.dumps({"operators_only": True, "extra": extra}), flush=True)
+        return
     ms = time_applies(lambda: A.ctx.a11(u, y, 0), args.steps, args.warmup, stream)
     gdofs = P.nvel / (ms * 1e-3) / 1e9
     bytes_alg = algorithmic_bytes_a11(m)
@@ -548,6 +581,10 @@ def main():
 
     if not args.apply_only:
         extra["hbm_copy_gbs"] = copy_bandwidth(stream)
+        try:
+            extra["operators"] = operator_timings(A.ctx, u, P, args.steps, args.warmup, stream)
+        except Exception as exc:  # reported, not fatal to the headline line
+            extra["operators"] = {"error": "%s: %s" % (type(exc).__name__, exc)}
     # release the apply set-up before the larger comparisons
     del A, u, y
     u32 = y32 = None
