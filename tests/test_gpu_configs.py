@@ -1,7 +1,8 @@
 """The benchmark workloads at their full sizes, checked against the oracle:
 
 C2  64^3 variable-viscosity A11 apply (12 inclusions eta 1e3 + log-uniform noise, affine+jitter
-    mesh), FP64 raw and MatMult at 1e-12, FP32 at 1e-5;
+    mesh), FP64 raw and MatMult at 1e-12, FP32 at 1e-5; the body-force load on the same mesh at
+    1e-12, and at 128^3 its integral of a constant load;
 C4  128^3 buoyancy solve (20 spheres, contrast 1e4, free-slip walls), 5 levels 128..8, FP64 and
     FP32 smoothing: the true residual, recomputed by the oracle's threaded apply, meets 1e-8;
 C5  8 buoyancy solves of uneven sizes (24^3 ... 80^3, contrasts 1e2 ... 1e6), concurrent equals
@@ -88,6 +89,43 @@ def test_c2_a11_fp32(c2):
     err = rel(y32, ref)
     say("  FP32 MatMult rel err %.3e" % err)
     assert err <= 1e-5
+    ctx.destroy()
+
+
+def test_c2_body_force(c2):
+    """The body-force load (FormFunctionLocal_U's Be term) on the C2 mesh against the oracle's
+    per-point loop, FP64 1e-12."""
+    P = c2
+    m = P.mx
+    say("case: C2 64^3 body force vs oracle")
+    fu = M.uniform(41, 81 * P.nel, -1.0, 1.0).reshape(-1, 27, 3)
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(m, m, m, P.coords, P.mask)
+    y = ctx.body_force(torch.from_numpy(fu).to(DEV)).cpu().numpy()
+    ref = O.body_force(m, m, m, P.coords, fu)
+    err = rel(y, ref)
+    say("  rel err %.3e" % err)
+    assert err <= 1e-12
+    ctx.destroy()
+
+
+@pytest.mark.timeout(600)
+def test_body_force_128_integrates_constants():
+    """At the 128^3 bench size, a size-independent property: for a constant load f, sum_n Be(n) =
+    f vol(Omega), with vol = |det SHEAR| for the sheared unit cube (interior jitter keeps the
+    boundary, and the 3-point rule integrates det J, of degree <= 5 per direction, exactly)."""
+    m = 128
+    say("case: body force 128^3, constant load")
+    c = M.affine(M.jitter(m, m, m, M.q2_uniform(m, m, m), 0.1, 1.0 / (2 * m), 5), M.SHEAR, [0.0, 0.0, 0.0])
+    ctx = ptq2.Context(0)
+    ctx.setup_mesh(m, m, m, c)
+    del c
+    f = torch.tensor([1.0, -2.0, 0.5], dtype=torch.float64, device=DEV)
+    fu = f.repeat(27 * m ** 3)
+    y = ctx.body_force(fu).view(-1, 3).sum(dim=0).cpu().numpy()
+    vol = abs(float(np.linalg.det(np.asarray(M.SHEAR, dtype=np.float64))))
+    say("  sums", y, "expected", vol * f.cpu().numpy())
+    assert np.allclose(y, vol * f.cpu().numpy(), rtol=1e-12, atol=0.0)
     ctx.destroy()
 
 
