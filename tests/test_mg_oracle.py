@@ -309,6 +309,43 @@ def _q1proj_brute(mx, my, mz, eta, nlevels):
     return out
 
 
+def test_q1_projection_hierarchy_reproduces_linear_fields_inside():
+    """What the mathematics fixes for the coefficient hierarchy: every step is a symmetric
+    weighted average (corner weights of the nodal projection mirror across the 8 elements of an
+    interior node, the restriction weights 1/2, 1, 1/2 are symmetric) or the Q1 interpolation,
+    which reproduces linear fields. So a field linear in space comes out exact at the coarse
+    quadrature points wherever no boundary node enters. In fine Q2-lattice units (node F at
+    F), fine element e spans [2e, 2e+2]; level-l element ex spans fine [2^(l+1) ex, 2^(l+1)(ex+1)]."""
+    m = 16
+    g = np.array([-np.sqrt(0.6), 0.0, np.sqrt(0.6)])
+    coef = np.array([5.0, 0.3, 0.2, 0.1])  # f = 5 + 0.3 X + 0.2 Y + 0.1 Z
+    e = np.arange(m)
+    q = np.arange(27)
+    pos = lambda el, gq, scale: scale * (2 * el[:, None] + 1 + gq[None, :])  # noqa: E731
+    X = pos(e, g[q % 3], 1)            # [ex, q]
+    Y = pos(e, g[(q // 3) % 3], 1)
+    Z = pos(e, g[q // 9], 1)
+    eta = (coef[0] + coef[1] * X[None, None, :, :] + coef[2] * Y[None, :, None, :]
+           + coef[3] * Z[:, None, None, :]).reshape(-1)
+    levels = G.q1_projection_hierarchy(m, m, m, eta, 3)
+    for lvl, vals in enumerate(levels, start=1):
+        mc = m >> lvl
+        sc = 2 ** lvl
+        ec = np.arange(mc)
+        Xc, Yc, Zc = pos(ec, g[q % 3], sc), pos(ec, g[(q // 3) % 3], sc), pos(ec, g[q // 9], sc)
+        want = (coef[0] + coef[1] * Xc[None, None, :, :] + coef[2] * Yc[None, :, None, :]
+                + coef[3] * Zc[:, None, None, :])
+        got = vals.reshape(mc, mc, mc, 27)
+        # elements whose coarse lattice nodes and restriction stencils stay off the boundary:
+        # level 1 nodes C in [1, m-1]; level 2 nodes D with 2D-1..2D+1 in [1, m-1]
+        lo, hi = 1, (mc * 2 - 3) // 2 if lvl == 1 else (m // 2 - 1 - 2) // 2
+        inner = slice(lo, hi + 1)
+        assert hi >= lo
+        assert np.allclose(got[inner, inner, inner], want[inner, inner, inner], rtol=1e-13, atol=0)
+        # and the boundary layer is not exact (the check above is not vacuous)
+        assert not np.allclose(got[0, 0, 0], want[0, 0, 0], rtol=1e-6, atol=0)
+
+
 def test_q1_projection_hierarchy_matches_loops_and_invariants():
     """The reference's coefficient hierarchy (coarse_eta_mean = q1proj): the vectorised
     restatement against the plain-loop one; constant fields stay exact; coarse values stay
