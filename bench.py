@@ -480,7 +480,7 @@ def main():
         args.solve = args.block = args.batch_apply = 0
         args.batch = False
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build runs on a single MI355X; --gpus %d is not supported"
+        print("bench.py measures one GPU only: the library runs on a single MI355X; --gpus %d is not supported"
               % args.gpus, file=sys.stderr)
         sys.exit(2)
 
