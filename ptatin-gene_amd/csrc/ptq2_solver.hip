@@ -93,6 +93,27 @@ __global__ void k_cheb_first(int64_t n, const T *__restrict__ b, const T *__rest
 // The strided read-modify-write of the x faces that dominates the separate pass goes away.
 // face_only: the apply's store already updated the Dirichlet dofs and every dof that no face
 // slot adds to (store_tile_own<true>); only the lower-face dofs are left.
+// four consecutive values through 16-byte accesses (the caller checks the alignment)
+template <typename T>
+__device__ __forceinline__ void ld4(const T *__restrict__ p, T v[4]) {
+  if constexpr (sizeof(T) == 4) {
+    const float4 a = *reinterpret_cast<const float4 *>(p);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+  } else {
+    const double2 a = reinterpret_cast<const double2 *>(p)[0], c = reinterpret_cast<const double2 *>(p)[1];
+    v[0] = a.x; v[1] = a.y; v[2] = c.x; v[3] = c.y;
+  }
+}
+template <typename T>
+__device__ __forceinline__ void st4(T *__restrict__ p, const T v[4]) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+  } else {
+    reinterpret_cast<double2 *>(p)[0] = make_double2(v[0], v[1]);
+    reinterpret_cast<double2 *>(p)[1] = make_double2(v[2], v[3]);
+  }
+}
+
 template <typename T>
 __global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T *__restrict__ y,
                              const T *__restrict__ dinv, const T *__restrict__ x, const T *__restrict__ pkm1,
@@ -103,7 +124,57 @@ __global__ void k_cheb_faces(Box bx, int64_t n, const T *__restrict__ b, const T
   const unsigned NX = (unsigned)bx.NX, NXY = (unsigned)bx.NX * (unsigned)bx.NY;
   const int64_t sx = ptq2::FACEVALS, sy = (int64_t)bx.nbx * ptq2::FACEVALS,
                 sz = (int64_t)bx.nbx * bx.nby * ptq2::FACEVALS;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  // the face slots added to the owner value v of dof i (a dof on a brick's lower face)
+  auto faced = [&](int64_t i, T v) -> T {
+    const unsigned nd = (unsigned)(i / 3), c = (unsigned)(i - 3 * (int64_t)nd);
+    const unsigned Z = nd / NXY, rem = nd - Z * NXY, Yn = rem / NX, X = rem - Yn * NX;
+    const int bxi = min((int)(X / NB), bx.nbx - 1), byi = min((int)(Yn / NB), bx.nby - 1),
+              bzi = min((int)(Z / NB), bx.nbz - 1);
+    const int lx = (int)X - NB * bxi, ly = (int)Yn - NB * byi, lz = (int)Z - NB * bzi;
+    const bool nx = lx == 0 && bxi > 0, ny = ly == 0 && byi > 0, nz = lz == 0 && bzi > 0;
+    if (!(nx || ny || nz)) return v;
+    const T *fb = face + ((int64_t)bxi + (int64_t)bx.nbx * (byi + (int64_t)bx.nby * bzi)) * ptq2::FACEVALS;
+    T acc = v;
+    if (nz && ny && nx) acc += fb[-sx - sy - sz + 3 * (UP * TN + UP) + c];
+    if (nz && ny) acc += fb[-sy - sz + 3 * (F2 + UP * TN + lx) + c];
+    if (nz && nx) acc += fb[-sx - sz + 3 * (UP * TN + ly) + c];
+    if (nz) acc += fb[-sz + 3 * (2 * F2 + ly * TN + lx) + c];
+    if (ny && nx) acc += fb[-sx - sy + 3 * (lz * TN + UP) + c];
+    if (ny) acc += fb[-sy + 3 * (F2 + lz * TN + lx) + c];
+    if (nx) acc += fb[-sx + 3 * (lz * TN + ly) + c];
+    return acc;
+  };
+  const bool al = (((uintptr_t)b | (uintptr_t)y | (uintptr_t)dinv | (uintptr_t)x | (uintptr_t)out |
+                    (uintptr_t)(pkm1 ? pkm1 : b)) & 15) == 0 && ((uintptr_t)(mask ? mask : (const uint8_t *)b) & 3) == 0;
+  int64_t i0 = 0;
+  // FP32 levels of 2M dofs and more (64^3 elements and up): every dof is updated, four
+  // consecutive dofs per step through 16-byte accesses.  Smaller levels keep one dof per thread
+  // (with a quarter of the threads their latency shows), and the FP64 update gains nothing.
+  if (sizeof(T) == 4 && !face_only && al && n >= (int64_t(1) << 21)) {
+    const int64_t nq = n / 4;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < nq; q += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t i = 4 * q;
+      T yv[4], xv[4], bv[4], dv[4], pv[4] = {T(0), T(0), T(0), T(0)}, ov[4];
+      ld4(y + i, yv);
+      ld4(x + i, xv);
+      ld4(b + i, bv);
+      ld4(dinv + i, dv);
+      if (pkm1) ld4(pkm1 + i, pv);
+      const uint32_t mk = mask ? *reinterpret_cast<const uint32_t *>(mask + i) : 0u;
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        T v;
+        if ((mk >> (8 * t)) & 0xffu)
+          v = xv[t];
+        else
+          v = face ? faced(i + t, yv[t]) : yv[t];
+        ov[t] = ptq2::cheb_value(co, dv[t], bv[t], v, xv[t], pv[t], pkm1 != nullptr);
+      }
+      st4(out + i, ov);
+    }
+    i0 = 4 * nq;
+  }
+  for (int64_t i = i0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     T yv;
     if (mask && mask[i]) {
       if (face_only) continue;
